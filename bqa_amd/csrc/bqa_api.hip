@@ -1,0 +1,1061 @@
+// libbqa_hip.so: C ABI of the MI355X BP quantum-annealing backend (include/bqa_hip.h).
+//
+// Host runtime: owns device buffers of one annealing job, sequences the
+// phase kernels on the job's stream, and keeps the BP convergence loop on the
+// device (a status block read back once per chunk of sweeps instead of once
+// per sweep, src/bqa/state.py:113).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bqa_hip.h"
+#include "common.hpp"
+#include "kernels_canon.hpp"
+#include "kernels_fast.hpp"
+#include "kernels_node.hpp"
+
+using namespace bqa;
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+struct BqaError : std::runtime_error {
+  int code;
+  BqaError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPX(expr)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw BqaError(e_ == hipErrorOutOfMemory ? BQA_ERR_MEMORY : BQA_ERR_HIP,            \
+                     std::string(#expr) + " failed: " + hipGetErrorString(e_));           \
+  } while (0)
+
+#define LAUNCH_CHECK() HIPX(hipGetLastError())
+
+void require(bool ok, const std::string& msg) {
+  if (!ok) throw BqaError(BQA_ERR_INVALID, msg);
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return BQA_OK;
+  } catch (const BqaError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return BQA_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return BQA_ERR_HIP;
+  }
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b, size_t& total, hipStream_t s = nullptr) {
+    if (b <= bytes) return;
+    if (s) HIPX(hipStreamSynchronize(s));
+    if (p) {
+      HIPX(hipFree(p));
+      total -= bytes;
+      p = nullptr;
+      bytes = 0;
+    }
+    if (b == 0) return;
+    HIPX(hipMalloc(&p, b));
+    bytes = b;
+    total += b;
+  }
+  void release(size_t& total) {
+    if (p) (void)hipFree(p);
+    total -= bytes;
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+double ipow_d(int b, int e) {
+  double r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+long ipow_l(int b, int e) {
+  long r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+}  // namespace
+
+struct ClassDev {
+  int d = 0, n = 0;
+  DevBuf layout;  // ids | in | out | lp (int32) | h | J (double)
+  int32_t *ids = nullptr, *in = nullptr, *out = nullptr, *lp = nullptr;
+  double *h = nullptr, *J = nullptr;
+  DevBuf T[2];
+  int tcur = 0;
+  ClassView view() const {
+    ClassView v;
+    v.n = n;
+    v.d = d;
+    v.node_ids = ids;
+    v.in_pos = in;
+    v.out_pos = out;
+    v.lmbd_pos = lp;
+    v.h = h;
+    v.J = J;
+    return v;
+  }
+};
+
+struct bqa_job {
+  int prec = BQA_PREC_SINGLE;
+  size_t csize = 8;  // bytes per complex element
+  int N = 0, M = 0, E = 0, chimax = 1, chi = 1;
+  int mcur = 0;
+  int lstride = 1;
+  int pred_sweeps = 8;
+  std::vector<ClassDev> cls;
+  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws;
+  BpStatus* st_host = nullptr;
+  double* lmax_host = nullptr;
+  hipStream_t stream = nullptr;
+  size_t dev_bytes = 0;
+  int device = 0;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+  int64_t t_launches = 0;
+  double t_ms = 0.0, t_bytes = 0.0;
+  std::vector<double> pending_bytes;
+
+  bool single() const { return prec == BQA_PREC_SINGLE; }
+};
+
+namespace {
+
+constexpr int kMaxLds = 64 * 1024;
+
+hipEvent_t take_event(bqa_job* j) {
+  if (!j->ev_pool.empty()) {
+    hipEvent_t e = j->ev_pool.back();
+    j->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIPX(hipEventCreate(&e));
+  return e;
+}
+
+void flush_timing(bqa_job* j) {
+  if (j->ev_pending.empty()) return;
+  HIPX(hipStreamSynchronize(j->stream));
+  for (size_t k = 0; k < j->ev_pending.size(); ++k) {
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, j->ev_pending[k].first, j->ev_pending[k].second));
+    j->t_ms += ms;
+    j->t_bytes += j->pending_bytes[k];
+    j->t_launches += 1;
+    j->ev_pool.push_back(j->ev_pending[k].first);
+    j->ev_pool.push_back(j->ev_pending[k].second);
+  }
+  j->ev_pending.clear();
+  j->pending_bytes.clear();
+}
+
+template <typename R>
+cplx<R>* msg_ptr(bqa_job* j, int which) {
+  return reinterpret_cast<cplx<R>*>(j->msgs[which].p);
+}
+
+// Tensor buffer sizes for bond dimension chi.
+size_t tensor_bytes(const bqa_job* j, const ClassDev& c, int chi) {
+  return (size_t)c.n * 2 * (size_t)ipow_l(chi, c.d) * j->csize;
+}
+
+void ensure_msgs(bqa_job* j, int chi) {
+  const size_t b = (size_t)j->M * chi * chi * j->csize;
+  // keep the current buffer's contents: grow both only when needed
+  if (b > j->msgs[0].bytes || b > j->msgs[1].bytes) {
+    HIPX(hipStreamSynchronize(j->stream));
+    for (int w = 0; w < 2; ++w) {
+      if (b <= j->msgs[w].bytes) continue;
+      void* np = nullptr;
+      HIPX(hipMalloc(&np, b));
+      if (j->msgs[w].p) {
+        HIPX(hipMemcpy(np, j->msgs[w].p, j->msgs[w].bytes, hipMemcpyDeviceToDevice));
+        HIPX(hipFree(j->msgs[w].p));
+        j->dev_bytes -= j->msgs[w].bytes;
+      }
+      j->msgs[w].p = np;
+      j->msgs[w].bytes = b;
+      j->dev_bytes += b;
+    }
+  }
+}
+
+void ensure_tensor(bqa_job* j, ClassDev& c, int which, int chi) {
+  const size_t b = tensor_bytes(j, c, chi);
+  if (b > c.T[which].bytes) {
+    HIPX(hipStreamSynchronize(j->stream));
+    c.T[which].ensure(b, j->dev_bytes);
+  }
+}
+
+// Grid and workspace of the generic node kernels: LDS when the three node
+// buffers fit, else a global slice per workgroup.
+struct NodeLaunch {
+  int grid;
+  size_t lds;
+  void* ws;
+  long ws_elems;
+};
+
+NodeLaunch plan_node(bqa_job* j, int n, size_t buf_bytes_total, size_t fixed_lds, size_t elems_per_block,
+                     size_t csize) {
+  NodeLaunch L;
+  if (fixed_lds + buf_bytes_total <= (size_t)kMaxLds) {
+    L.grid = std::max(1, std::min(n, 4096));
+    L.lds = fixed_lds + buf_bytes_total;
+    L.ws = nullptr;
+    L.ws_elems = 0;
+  } else {
+    L.grid = std::max(1, std::min(n, 1024));
+    const size_t need = (size_t)L.grid * elems_per_block * csize;
+    if (need > j->gws.bytes) {
+      j->gws.ensure(need, j->dev_bytes, j->stream);
+    }
+    L.lds = fixed_lds;
+    L.ws = j->gws.p;
+    L.ws_elems = (long)elems_per_block;
+  }
+  return L;
+}
+
+// Algorithmic bytes of one BP sweep over one class (the roofline numerator):
+// tensors read once, incoming and old messages read, new messages written,
+// two int32 position arrays.
+double sweep_bytes(const bqa_job* j, const ClassDev& c, int chi) {
+  const double cs = (double)j->csize;
+  return (double)c.n * 2.0 * ipow_d(chi, c.d) * cs + 3.0 * c.n * c.d * (double)chi * chi * cs +
+         (double)c.n * c.d * 8.0;
+}
+
+__global__ void k_bp_reset(BpStatus* st, int cur, int max_iters, double bp_eps) {
+  st->num_bits = 0ULL;
+  st->den_bits = 0ULL;
+  st->cur = cur;
+  st->done = 0;
+  st->converged = 0;
+  st->iters = 0;
+  st->have_best = 0;
+  st->damp_now = 0;
+  st->max_iters = max_iters;
+  st->bp_eps = bp_eps;
+  st->last_dist = 0.0;
+  st->best_dist = INFINITY;
+}
+
+// After every sweep: distance, convergence and buffer bookkeeping
+// (src/bqa/state.py:113-121).
+__global__ void k_bp_finish(BpStatus* st, int damping_on) {
+  if (st->done) return;
+  const double num = __longlong_as_double((long long)st->num_bits);
+  const double den = __longlong_as_double((long long)st->den_bits);
+  st->num_bits = 0ULL;
+  st->den_bits = 0ULL;
+  const double dist = num / den;
+  st->iters += 1;
+  st->last_dist = dist;
+  if (dist < st->best_dist) {
+    st->best_dist = dist;
+    st->have_best = 1;
+  }
+  st->damp_now = 0;
+  if (dist < st->bp_eps) {
+    st->converged = 1;
+    st->done = 1;  // keep the previous iterate
+  } else if (st->iters >= st->max_iters) {
+    st->done = 1;
+    st->cur = 1 - st->cur;  // the last undamped iterate
+  } else if (damping_on) {
+    st->damp_now = 1;
+  } else {
+    st->cur = 1 - st->cur;
+  }
+}
+
+// msgs[cur] <- a msgs[cur] + (1 - a) msgs[1 - cur] (backends.py:761-764).
+template <typename R>
+__global__ void k_bp_damp(BpStatus* st, cplx<R>* m0, cplx<R>* m1, long count, double alpha) {
+  if (!st->damp_now) return;
+  cplx<R>* dst = st->cur ? m1 : m0;
+  const cplx<R>* src = st->cur ? m0 : m1;
+  const R a = (R)alpha, b = (R)(1.0 - alpha);
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < count; x += (long)gridDim.x * blockDim.x) {
+    const cplx<R> v = dst[x], w = src[x];
+    dst[x] = mk<R>(v.x * a + b * w.x, v.y * a + b * w.y);
+  }
+}
+
+template <typename R>
+__global__ void k_init_tensors(cplx<R>* T, long n) {
+  const R h = (R)0.70710678118654752440;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) {
+    T[2 * x] = mk<R>(h, 0);
+    T[2 * x + 1] = mk<R>(-h, 0);
+  }
+}
+
+template <typename R>
+__global__ void k_fill_one(cplx<R>* m, long n) {
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
+    m[x] = mk<R>(1, 0);
+}
+
+__global__ void k_fill_d(double* p, long n, double v) {
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x) p[x] = v;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  for (int x = threadIdx.x; x < n; x += blockDim.x) p[x] = 0ULL;
+}
+
+// Energy of the rounded spins: partial sums per block, deterministic order.
+__global__ void k_energy_partial(const double* bloch, int N, const double* h_by_id, const int32_t* lhs,
+                                 const int32_t* rhs, const double* jv, int E, double* part) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < N; x += (long)gridDim.x * blockDim.x) {
+    const double s = bloch[x * 3 + 2] > 0.0 ? 1.0 : -1.0;
+    acc += h_by_id[x] * s;
+  }
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < E; x += (long)gridDim.x * blockDim.x) {
+    const double sa = bloch[(long)lhs[x] * 3 + 2] > 0.0 ? 1.0 : -1.0;
+    const double sb = bloch[(long)rhs[x] * 3 + 2] > 0.0 ? 1.0 : -1.0;
+    acc += jv[x] * sa * sb;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void k_energy_final(const double* part, int nb, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    *out = s;
+  }
+}
+
+__global__ void k_copy_bytes(const uint4* __restrict__ src, uint4* __restrict__ dst, long n) {
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
+    dst[x] = src[x];
+}
+
+int grid_for(long n, int block = 256, int cap = 8192) {
+  long g = (n + block - 1) / block;
+  return (int)std::max<long>(1, std::min<long>(g, cap));
+}
+
+// ---------------------------------------------------------------- phases --
+
+template <typename R>
+void launch_bp_class(bqa_job* j, ClassDev& c) {
+  const int chi = j->chi;
+  if (c.d == 0 || c.n == 0) return;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (j->timing) {
+    e0 = take_event(j);
+    e1 = take_event(j);
+    HIPX(hipEventRecord(e0, j->stream));
+  }
+  if (!fast_bp_launch<R>(c.view(), chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
+                         msg_ptr<R>(j, 1), reinterpret_cast<BpStatus*>(j->status.p), j->stream)) {
+    const long S = 2 * ipow_l(chi, c.d);
+    const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
+    NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 3 * S, fixed, 3 * S, sizeof(cplx<R>));
+    hipLaunchKernelGGL((k_node_contract<R, kModeBP>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi,
+                       reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0), msg_ptr<R>(j, 1),
+                       reinterpret_cast<BpStatus*>(j->status.p), (cplx<R>*)nullptr, 0.0, (double*)nullptr,
+                       (double*)nullptr, reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
+  }
+  LAUNCH_CHECK();
+  if (j->timing) {
+    HIPX(hipEventRecord(e1, j->stream));
+    j->ev_pending.emplace_back(e0, e1);
+    j->pending_bytes.push_back(sweep_bytes(j, c, chi));
+  }
+}
+
+template <typename R>
+void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* sweeps, int* converged,
+                 double* last_dist) {
+  require(max_iters > 0, "max_bp_iter_number must be positive: BP needs at least one sweep (state.py:122)");
+  BpStatus* st = reinterpret_cast<BpStatus*>(j->status.p);
+  hipLaunchKernelGGL(k_bp_reset, dim3(1), dim3(1), 0, j->stream, st, j->mcur, max_iters, bp_eps);
+  LAUNCH_CHECK();
+  const long count = (long)j->M * j->chi * j->chi;
+  const int damping_on = damping > 0.0 ? 1 : 0;
+  const size_t ev_start = j->ev_pending.size();
+  int active_classes = 0;
+  for (auto& cl : j->cls) active_classes += (cl.d > 0 && cl.n > 0) ? 1 : 0;
+  int launched = 0;
+  int chunk = std::max(1, std::min(j->pred_sweeps, max_iters));
+  while (launched < max_iters) {
+    const int c = std::min(chunk, max_iters - launched);
+    for (int s = 0; s < c; ++s) {
+      for (auto& cl : j->cls) launch_bp_class<R>(j, cl);
+      hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, j->stream, st, damping_on);
+      LAUNCH_CHECK();
+      if (damping_on) {
+        hipLaunchKernelGGL((k_bp_damp<R>), dim3(grid_for(count)), dim3(256), 0, j->stream, st, msg_ptr<R>(j, 0),
+                           msg_ptr<R>(j, 1), count, damping);
+        LAUNCH_CHECK();
+      }
+    }
+    launched += c;
+    HIPX(hipMemcpyAsync(j->st_host, st, sizeof(BpStatus), hipMemcpyDeviceToHost, j->stream));
+    HIPX(hipStreamSynchronize(j->stream));
+    if (j->st_host->done) break;
+    chunk = 4;
+  }
+  if (j->timing) {
+    // launches after convergence exit at once: time only the sweeps that ran
+    const size_t keep = ev_start + (size_t)j->st_host->iters * active_classes;
+    while (j->ev_pending.size() > keep) {
+      j->ev_pool.push_back(j->ev_pending.back().first);
+      j->ev_pool.push_back(j->ev_pending.back().second);
+      j->ev_pending.pop_back();
+      j->pending_bytes.pop_back();
+    }
+    flush_timing(j);
+  }
+  const BpStatus& h = *j->st_host;
+  j->mcur = h.cur;
+  j->pred_sweeps = std::max(1, h.iters);
+  *sweeps = h.iters;
+  *converged = h.converged;
+  *last_dist = h.last_dist;
+  if (!h.have_best) throw BqaError(BQA_ERR_NUMERIC, "BP produced no finite message distance");
+}
+
+template <typename R>
+void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
+  const int chi = j->chi, n2 = 2 * chi;
+  j->ext.ensure((size_t)j->M * n2 * n2 * j->csize, j->dev_bytes, j->stream);
+  for (auto& c : j->cls) {
+    if (c.d == 0 || c.n == 0) continue;
+    const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
+    cplx<R>* ext = reinterpret_cast<cplx<R>*>(j->ext.p);
+    if (fast_ext_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), ext, ztime, j->stream)) {
+      LAUNCH_CHECK();
+      continue;
+    }
+    const long S = 2 * ipow_l(chi, c.d);
+    const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
+    NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 3 * S, fixed, 3 * S, sizeof(cplx<R>));
+    hipLaunchKernelGGL((k_node_contract<R, kModeExt>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi, T,
+                       msg_ptr<R>(j, j->mcur), (cplx<R>*)nullptr, (BpStatus*)nullptr, ext, ztime, (double*)nullptr,
+                       (double*)nullptr, reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
+    LAUNCH_CHECK();
+  }
+  const int ccols = std::min(n2, j->chimax);
+  j->canon.ensure((size_t)j->M * n2 * ccols * sizeof(zd), j->dev_bytes, j->stream);
+  j->lmbd.ensure((size_t)j->E * n2 * sizeof(double), j->dev_bytes, j->stream);
+  const size_t wsb = canon_ws_bytes(n2);
+  const double deps = j->single() ? (double)std::numeric_limits<float>::epsilon()
+                                  : std::numeric_limits<double>::epsilon();
+  if (!fast_canon_launch<R>(reinterpret_cast<const cplx<R>*>(j->ext.p), j->E, chi, ccols, pinv_eps, deps,
+                            reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p), j->stream)) {
+    int grid;
+    size_t lds;
+    unsigned char* ws = nullptr;
+    if (wsb <= (size_t)kMaxLds) {
+      grid = std::max(1, std::min(j->E, 16384));
+      lds = wsb;
+    } else {
+      grid = std::max(1, std::min(j->E, 2048));
+      j->cws.ensure((size_t)grid * wsb, j->dev_bytes, j->stream);
+      ws = reinterpret_cast<unsigned char*>(j->cws.p);
+      lds = 0;
+    }
+    hipLaunchKernelGGL((k_canonicalize<R>), dim3(grid), dim3(kCanonNT), lds, j->stream,
+                       reinterpret_cast<const cplx<R>*>(j->ext.p), j->E, chi, ccols, pinv_eps, deps,
+                       reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p), ws, (long)wsb);
+    LAUNCH_CHECK();
+  }
+  unsigned long long* lm = reinterpret_cast<unsigned long long*>(j->lmax.p);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, j->stream, lm, n2);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_lambda_max, dim3(grid_for((long)j->E * n2, 256, 1024)), dim3(256), 0, j->stream,
+                     reinterpret_cast<const double*>(j->lmbd.p), j->E, n2, lm);
+  LAUNCH_CHECK();
+  HIPX(hipMemcpyAsync(j->lmax_host, lm, sizeof(double) * n2, hipMemcpyDeviceToHost, j->stream));
+  HIPX(hipStreamSynchronize(j->stream));
+}
+
+// Global truncation rule (backends.py:297-303).
+int truncation_dim(const bqa_job* j, int n2, double pinv_eps, double* err) {
+  int below = 0;
+  for (int k = 0; k < n2; ++k) below += j->lmax_host[k] < pinv_eps ? 1 : 0;
+  const int rank = n2 - below;
+  const int dim = std::min(rank, j->chimax);
+  double e = 0.0;
+  for (int k = dim; k < n2; ++k) e += j->lmax_host[k] * j->lmax_host[k];
+  *err = std::sqrt(e);
+  return dim;
+}
+
+template <typename R>
+void local_update(bqa_job* j, int flags, int chi_out, double zz_time, double z_time, double x_time) {
+  const int chi_in = j->chi;
+  const int n2 = 2 * chi_in;
+  const int ccols = std::min(n2, j->chimax);
+  if (flags & kGauge) ensure_msgs(j, chi_out);
+  for (auto& c : j->cls) {
+    if (c.n == 0) continue;
+    const int dst = 1 - c.tcur;
+    ensure_tensor(j, c, dst, chi_out);
+    const cplx<R>* Tin = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
+    cplx<R>* Tout = reinterpret_cast<cplx<R>*>(c.T[dst].p);
+    if (!fast_local_launch<R>(c.view(), chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
+                              n2, ccols, zz_time, flags, z_time, x_time,
+                              reinterpret_cast<const double*>(j->lmbd.p), j->lstride, msg_ptr<R>(j, j->mcur),
+                              j->stream)) {
+      const int cm = std::max(chi_in, chi_out);
+      const long Smax = 2 * ipow_l(cm, c.d);
+      NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 2 * Smax, 0, 2 * Smax, sizeof(cplx<R>));
+      hipLaunchKernelGGL((k_node_local<R>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi_in, chi_out,
+                         Tin, Tout, reinterpret_cast<const zd*>(j->canon.p), n2, ccols, zz_time, flags, z_time,
+                         x_time, reinterpret_cast<const double*>(j->lmbd.p), j->lstride, msg_ptr<R>(j, j->mcur),
+                         reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
+    }
+    LAUNCH_CHECK();
+    c.tcur = dst;
+  }
+  j->chi = chi_out;
+}
+
+template <typename R>
+void simple_update_impl(bqa_job* j, double ztime, double pinv_eps, int extra_flags, double z_time,
+                        double x_time, int* new_dim, double* err) {
+  ext_and_canon<R>(j, ztime, pinv_eps);
+  const int n2 = 2 * j->chi;
+  const int dim = truncation_dim(j, n2, pinv_eps, err);
+  j->lstride = n2;
+  local_update<R>(j, kAbsorb | extra_flags, dim, ztime, z_time, x_time);
+  *new_dim = dim;
+}
+
+template <typename R>
+void density_impl(bqa_job* j, double* rho_out, double* bloch_out) {
+  j->rho.ensure((size_t)j->N * 8 * sizeof(double), j->dev_bytes, j->stream);
+  j->bloch.ensure((size_t)j->N * 3 * sizeof(double), j->dev_bytes, j->stream);
+  const int chi = j->chi;
+  for (auto& c : j->cls) {
+    if (c.n == 0) continue;
+    const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
+    if (fast_density_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), reinterpret_cast<double*>(j->rho.p),
+                               reinterpret_cast<double*>(j->bloch.p), j->stream)) {
+      LAUNCH_CHECK();
+      continue;
+    }
+    const long S = 2 * ipow_l(chi, c.d);
+    const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
+    NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 3 * S, fixed, 3 * S, sizeof(cplx<R>));
+    hipLaunchKernelGGL((k_node_contract<R, kModeDensity>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi,
+                       T, msg_ptr<R>(j, j->mcur), (cplx<R>*)nullptr, (BpStatus*)nullptr, (cplx<R>*)nullptr, 0.0,
+                       reinterpret_cast<double*>(j->rho.p), reinterpret_cast<double*>(j->bloch.p),
+                       reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
+    LAUNCH_CHECK();
+  }
+  if (rho_out)
+    HIPX(hipMemcpyAsync(rho_out, j->rho.p, (size_t)j->N * 8 * sizeof(double), hipMemcpyDeviceToHost, j->stream));
+  if (bloch_out)
+    HIPX(hipMemcpyAsync(bloch_out, j->bloch.p, (size_t)j->N * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                        j->stream));
+  HIPX(hipStreamSynchronize(j->stream));
+}
+
+template <typename R>
+void init_state_impl(bqa_job* j) {
+  j->chi = 1;
+  j->mcur = 0;
+  j->lstride = 1;
+  ensure_msgs(j, 1);
+  for (auto& c : j->cls) {
+    c.tcur = 0;
+    if (c.n == 0) continue;
+    ensure_tensor(j, c, 0, 1);
+    hipLaunchKernelGGL((k_init_tensors<R>), dim3(grid_for(c.n)), dim3(256), 0, j->stream,
+                       reinterpret_cast<cplx<R>*>(c.T[0].p), (long)c.n);
+    LAUNCH_CHECK();
+  }
+  j->lmbd.ensure((size_t)j->E * 2 * sizeof(double), j->dev_bytes, j->stream);
+  hipLaunchKernelGGL(k_fill_d, dim3(grid_for(j->E)), dim3(256), 0, j->stream, reinterpret_cast<double*>(j->lmbd.p),
+                     (long)j->E, 1.0);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_fill_one<R>), dim3(grid_for(j->M)), dim3(256), 0, j->stream, msg_ptr<R>(j, 0), (long)j->M);
+  LAUNCH_CHECK();
+  HIPX(hipStreamSynchronize(j->stream));
+  j->pred_sweeps = 8;
+}
+
+// ------------------------------------------------------- state transfer --
+
+template <typename R>
+void download(bqa_job* j, const void* dev, size_t count, double* out) {
+  std::vector<cplx<R>> h(count);
+  HIPX(hipMemcpyAsync(h.data(), dev, count * sizeof(cplx<R>), hipMemcpyDeviceToHost, j->stream));
+  HIPX(hipStreamSynchronize(j->stream));
+  for (size_t x = 0; x < count; ++x) {
+    out[2 * x] = (double)h[x].x;
+    out[2 * x + 1] = (double)h[x].y;
+  }
+}
+
+template <typename R>
+void upload(bqa_job* j, void* dev, size_t count, const double* in) {
+  std::vector<cplx<R>> h(count);
+  for (size_t x = 0; x < count; ++x) h[x] = mk<R>((R)in[2 * x], (R)in[2 * x + 1]);
+  HIPX(hipMemcpyAsync(dev, h.data(), count * sizeof(cplx<R>), hipMemcpyHostToDevice, j->stream));
+  HIPX(hipStreamSynchronize(j->stream));
+}
+
+template <typename F>
+void dispatch(bqa_job* j, F&& f) {
+  HIPX(hipSetDevice(j->device));
+  if (j->single())
+    f(float());
+  else
+    f(double());
+}
+
+size_t estimate(int M, const std::vector<int>& deg, const std::vector<int>& size, int chimax, int prec) {
+  const size_t cs = prec == BQA_PREC_SINGLE ? 8 : 16;
+  double b = 0.0;
+  for (size_t c = 0; c < deg.size(); ++c) b += 2.0 * size[c] * 2.0 * ipow_d(chimax, deg[c]) * cs;
+  b += 2.0 * M * (double)chimax * chimax * cs;                  // messages
+  b += (double)M * 4.0 * chimax * chimax * cs;                  // extended messages
+  b += (double)M * 2.0 * chimax * chimax * sizeof(zd);          // canonicalisers
+  b += (double)(M / 2) * 2.0 * chimax * sizeof(double);         // lambdas
+  return (size_t)b;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char* bqa_last_error(void) { return g_last_error.c_str(); }
+
+int bqa_device_info(int* device_count, int* arch_ok, char* name_out, int name_len) {
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (device_count) *device_count = n;
+    if (arch_ok) *arch_ok = 0;
+    if (n > 0) {
+      hipDeviceProp_t p;
+      HIPX(hipGetDeviceProperties(&p, 0));
+      if (arch_ok) *arch_ok = std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+      if (name_out && name_len > 0) {
+        std::snprintf(name_out, name_len, "%s", p.gcnArchName);
+      }
+    }
+  });
+}
+
+int bqa_job_estimate_bytes(int32_t edges_number, int32_t classes, const int32_t* class_degree,
+                           const int32_t* class_size, int32_t max_bond_dim, int32_t precision, uint64_t* bytes) {
+  return guarded([&] {
+    require(bytes && class_degree && class_size && classes >= 0, "null argument");
+    std::vector<int> deg(class_degree, class_degree + classes), sz(class_size, class_size + classes);
+    *bytes = estimate(edges_number, deg, sz, max_bond_dim, precision);
+  });
+}
+
+int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, int32_t classes,
+                   const int32_t* class_degree, const int32_t* class_size, const int32_t* node_ids,
+                   const int32_t* input_pos, const int32_t* output_pos, const int32_t* lmbd_pos,
+                   const double* node_ampls, const double* edge_ampls, const int32_t* fwd_lhs,
+                   const int32_t* fwd_rhs, const double* fwd_j, int32_t max_bond_dim, int32_t precision) {
+  bqa_job* j = nullptr;
+  int rc = guarded([&] {
+    require(out != nullptr, "null job pointer");
+    require(nodes_number > 0 && edges_number > 0 && edges_number % 2 == 0, "need n > 0 and an even number of messages");
+    require(classes > 0 && class_degree && class_size, "no degree classes");
+    require(max_bond_dim >= 1 && max_bond_dim <= 32, "max_bond_dim must be in [1, 32]");
+    require(precision == BQA_PREC_SINGLE || precision == BQA_PREC_DOUBLE, "unknown precision");
+    int dev_count = 0;
+    HIPX(hipGetDeviceCount(&dev_count));
+    require(dev_count > 0, "no HIP device");
+    j = new bqa_job();
+    j->prec = precision;
+    j->csize = precision == BQA_PREC_SINGLE ? 8 : 16;
+    j->N = nodes_number;
+    j->M = edges_number;
+    j->E = edges_number / 2;
+    j->chimax = max_bond_dim;
+    HIPX(hipGetDevice(&j->device));
+    HIPX(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    long total_nodes = 0, total_bonds = 0;
+    for (int c = 0; c < classes; ++c) {
+      require(class_degree[c] >= 0 && class_size[c] >= 0, "negative class degree or size");
+      total_nodes += class_size[c];
+      total_bonds += (long)class_degree[c] * class_size[c];
+    }
+    require(total_nodes == nodes_number, "class sizes do not add up to nodes_number");
+    require(total_bonds == edges_number, "class degrees do not add up to edges_number");
+    long off_n = 0, off_b = 0;
+    std::vector<double> h_by_id(nodes_number, 0.0);
+    j->cls.resize(classes);
+    for (int c = 0; c < classes; ++c) {
+      ClassDev& cl = j->cls[c];
+      cl.d = class_degree[c];
+      cl.n = class_size[c];
+      require(ipow_d(max_bond_dim, cl.d) * 2.0 < 2147483647.0, "degree too large for max_bond_dim");
+      const size_t nb = (size_t)cl.d * cl.n;
+      const size_t ib = sizeof(int32_t) * ((size_t)cl.n + 3 * nb);
+      const size_t db = sizeof(double) * ((size_t)cl.n + nb);
+      cl.layout.ensure(ib + db + 64, j->dev_bytes);
+      char* base = reinterpret_cast<char*>(cl.layout.p);
+      cl.ids = reinterpret_cast<int32_t*>(base);
+      cl.in = cl.ids + cl.n;
+      cl.out = cl.in + nb;
+      cl.lp = cl.out + nb;
+      cl.h = reinterpret_cast<double*>(base + ((ib + 15) / 16) * 16);
+      cl.J = cl.h + cl.n;
+      for (long x = 0; x < cl.n; ++x) {
+        const int id = node_ids[off_n + x];
+        require(id >= 0 && id < nodes_number, "node id out of range");
+        h_by_id[id] = node_ampls[off_n + x];
+      }
+      for (size_t x = 0; x < nb; ++x) {
+        require(input_pos[off_b + x] >= 0 && input_pos[off_b + x] < edges_number, "input position out of range");
+        require(output_pos[off_b + x] >= 0 && output_pos[off_b + x] < edges_number, "output position out of range");
+        require(lmbd_pos[off_b + x] >= 0 && lmbd_pos[off_b + x] < edges_number / 2, "lambda position out of range");
+        require(lmbd_pos[off_b + x] == input_pos[off_b + x] % (edges_number / 2),
+                "lambda position must equal the message position mod E");
+      }
+      HIPX(hipMemcpy(cl.ids, node_ids + off_n, sizeof(int32_t) * cl.n, hipMemcpyHostToDevice));
+      if (nb) {
+        HIPX(hipMemcpy(cl.in, input_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(cl.out, output_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(cl.lp, lmbd_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(cl.J, edge_ampls + off_b, sizeof(double) * nb, hipMemcpyHostToDevice));
+      }
+      HIPX(hipMemcpy(cl.h, node_ampls + off_n, sizeof(double) * cl.n, hipMemcpyHostToDevice));
+      off_n += cl.n;
+      off_b += nb;
+    }
+    const int E = j->E;
+    j->edges.ensure(sizeof(int32_t) * 2 * E + sizeof(double) * (E + (size_t)nodes_number) + 64, j->dev_bytes);
+    {
+      char* base = reinterpret_cast<char*>(j->edges.p);
+      int32_t* lhs = reinterpret_cast<int32_t*>(base);
+      int32_t* rhs = lhs + E;
+      double* jv = reinterpret_cast<double*>(base + ((sizeof(int32_t) * 2 * E + 15) / 16) * 16);
+      double* hv = jv + E;
+      for (int x = 0; x < E; ++x) {
+        require(fwd_lhs[x] >= 0 && fwd_lhs[x] < nodes_number && fwd_rhs[x] >= 0 && fwd_rhs[x] < nodes_number,
+                "edge endpoint out of range");
+      }
+      HIPX(hipMemcpy(lhs, fwd_lhs, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(rhs, fwd_rhs, sizeof(int32_t) * E, hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(jv, fwd_j, sizeof(double) * E, hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(hv, h_by_id.data(), sizeof(double) * nodes_number, hipMemcpyHostToDevice));
+    }
+    j->status.ensure(sizeof(BpStatus), j->dev_bytes);
+    j->lmax.ensure(sizeof(double) * 128, j->dev_bytes);
+    j->epart.ensure(sizeof(double) * 1024 + 64, j->dev_bytes);
+    HIPX(hipHostMalloc(reinterpret_cast<void**>(&j->st_host), sizeof(BpStatus), hipHostMallocDefault));
+    HIPX(hipHostMalloc(reinterpret_cast<void**>(&j->lmax_host), sizeof(double) * 128, hipHostMallocDefault));
+    *out = j;
+  });
+  if (rc != BQA_OK && j) bqa_job_destroy(j);
+  return rc;
+}
+
+int bqa_job_destroy(bqa_job* j) {
+  if (!j) return BQA_OK;
+  return guarded([&] {
+    (void)hipSetDevice(j->device);
+    if (j->stream) (void)hipStreamSynchronize(j->stream);
+    for (auto& c : j->cls) {
+      c.layout.release(j->dev_bytes);
+      c.T[0].release(j->dev_bytes);
+      c.T[1].release(j->dev_bytes);
+    }
+    DevBuf* bufs[] = {&j->msgs[0], &j->msgs[1], &j->lmbd, &j->ext, &j->canon, &j->lmax, &j->status,
+                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws};
+    for (DevBuf* b : bufs) b->release(j->dev_bytes);
+    for (auto& pr : j->ev_pending) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    for (auto e : j->ev_pool) (void)hipEventDestroy(e);
+    if (j->st_host) (void)hipHostFree(j->st_host);
+    if (j->lmax_host) (void)hipHostFree(j->lmax_host);
+    if (j->stream) (void)hipStreamDestroy(j->stream);
+    delete j;
+  });
+}
+
+int bqa_job_device_bytes(const bqa_job* j, uint64_t* bytes) {
+  return guarded([&] {
+    require(j && bytes, "null argument");
+    *bytes = j->dev_bytes;
+  });
+}
+
+int bqa_init_state(bqa_job* j) {
+  return guarded([&] {
+    require(j, "null job");
+    dispatch(j, [&](auto r) { init_state_impl<decltype(r)>(j); });
+  });
+}
+
+int bqa_run_bp(bqa_job* j, int32_t max_iters, double bp_eps, double damping, int32_t* sweeps, int32_t* converged,
+               double* last_dist) {
+  return guarded([&] {
+    require(j && sweeps && converged && last_dist, "null argument");
+    require(damping >= 0.0 && damping <= 1.0, "damping must be in [0, 1]");
+    dispatch(j, [&](auto r) {
+      int s = 0, c = 0;
+      double d = 0.0;
+      run_bp_impl<decltype(r)>(j, max_iters, bp_eps, damping, &s, &c, &d);
+      *sweeps = s;
+      *converged = c;
+      *last_dist = d;
+    });
+  });
+}
+
+int bqa_simple_update(bqa_job* j, double ztime, double pinv_eps, int32_t* new_bond_dim, double* trunc_err) {
+  return guarded([&] {
+    require(j && new_bond_dim && trunc_err, "null argument");
+    dispatch(j, [&](auto r) {
+      int dim = 0;
+      simple_update_impl<decltype(r)>(j, ztime, pinv_eps, 0, 0.0, 0.0, &dim, trunc_err);
+      *new_bond_dim = dim;
+    });
+  });
+}
+
+int bqa_apply_local_gates(bqa_job* j, double ztime, double xtime) {
+  return guarded([&] {
+    require(j, "null job");
+    dispatch(j, [&](auto r) { local_update<decltype(r)>(j, kGates, j->chi, 0.0, ztime, xtime); });
+  });
+}
+
+int bqa_set_symmetric_gauge(bqa_job* j) {
+  return guarded([&] {
+    require(j, "null job");
+    dispatch(j, [&](auto r) { local_update<decltype(r)>(j, kGauge, j->chi, 0.0, 0.0, 0.0); });
+  });
+}
+
+int bqa_run_layer(bqa_job* j, double xtime, double ztime, double pinv_eps, int32_t max_iters, double bp_eps,
+                  double damping, int32_t* new_bond_dim, double* trunc_err, int32_t* sweeps, int32_t* converged,
+                  double* last_dist) {
+  return guarded([&] {
+    require(j && new_bond_dim && trunc_err && sweeps && converged && last_dist, "null argument");
+    require(damping >= 0.0 && damping <= 1.0, "damping must be in [0, 1]");
+    dispatch(j, [&](auto r) {
+      using R = decltype(r);
+      int dim = 0;
+      simple_update_impl<R>(j, ztime, pinv_eps, kGates | kGauge, ztime, xtime, &dim, trunc_err);
+      *new_bond_dim = dim;
+      int s = 0, c = 0;
+      double d = 0.0;
+      run_bp_impl<R>(j, max_iters, bp_eps, damping, &s, &c, &d);
+      *sweeps = s;
+      *converged = c;
+      *last_dist = d;
+    });
+  });
+}
+
+int bqa_density(bqa_job* j, double* rho_out, double* bloch_out) {
+  return guarded([&] {
+    require(j, "null job");
+    dispatch(j, [&](auto r) { density_impl<decltype(r)>(j, rho_out, bloch_out); });
+  });
+}
+
+int bqa_energy(bqa_job* j, double* energy, int8_t* spins_out) {
+  return guarded([&] {
+    require(j && energy, "null argument");
+    require(j->bloch.p != nullptr, "bqa_density must run before bqa_energy");
+    HIPX(hipSetDevice(j->device));
+    const int E = j->E;
+    char* base = reinterpret_cast<char*>(j->edges.p);
+    const int32_t* lhs = reinterpret_cast<const int32_t*>(base);
+    const int32_t* rhs = lhs + E;
+    const double* jv = reinterpret_cast<const double*>(base + ((sizeof(int32_t) * 2 * E + 15) / 16) * 16);
+    const double* hv = jv + E;
+    double* part = reinterpret_cast<double*>(j->epart.p);
+    const int nb = 512;
+    hipLaunchKernelGGL(k_energy_partial, dim3(nb), dim3(256), 0, j->stream,
+                       reinterpret_cast<const double*>(j->bloch.p), j->N, hv, lhs, rhs, jv, E, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, j->stream, part, nb, part + nb);
+    LAUNCH_CHECK();
+    HIPX(hipMemcpyAsync(energy, part + nb, sizeof(double), hipMemcpyDeviceToHost, j->stream));
+    if (spins_out) {
+      std::vector<double> b((size_t)j->N * 3);
+      HIPX(hipMemcpyAsync(b.data(), j->bloch.p, b.size() * sizeof(double), hipMemcpyDeviceToHost, j->stream));
+      HIPX(hipStreamSynchronize(j->stream));
+      for (int v = 0; v < j->N; ++v) spins_out[v] = b[(size_t)v * 3 + 2] > 0.0 ? 1 : -1;
+    }
+    HIPX(hipStreamSynchronize(j->stream));
+  });
+}
+
+int bqa_bond_dim(const bqa_job* j, int32_t* chi) {
+  return guarded([&] {
+    require(j && chi, "null argument");
+    *chi = j->chi;
+  });
+}
+
+int bqa_get_tensors(bqa_job* j, int32_t ci, double* out) {
+  return guarded([&] {
+    require(j && out && ci >= 0 && ci < (int)j->cls.size(), "bad class index");
+    ClassDev& c = j->cls[ci];
+    const size_t count = (size_t)c.n * 2 * ipow_l(j->chi, c.d);
+    if (count == 0) return;
+    dispatch(j, [&](auto r) { download<decltype(r)>(j, c.T[c.tcur].p, count, out); });
+  });
+}
+
+int bqa_set_tensors(bqa_job* j, int32_t ci, int32_t chi, const double* in) {
+  return guarded([&] {
+    require(j && in && ci >= 0 && ci < (int)j->cls.size(), "bad class index");
+    require(chi >= 1 && chi <= j->chimax, "bond dimension out of range");
+    // every class shares the job's bond dimension: callers set all classes
+    ClassDev& c = j->cls[ci];
+    ensure_tensor(j, c, c.tcur, chi);
+    const size_t count = (size_t)c.n * 2 * ipow_l(chi, c.d);
+    if (count) dispatch(j, [&](auto r) { upload<decltype(r)>(j, c.T[c.tcur].p, count, in); });
+    j->chi = chi;
+  });
+}
+
+int bqa_get_messages(bqa_job* j, double* out) {
+  return guarded([&] {
+    require(j && out, "null argument");
+    const size_t count = (size_t)j->M * j->chi * j->chi;
+    dispatch(j, [&](auto r) { download<decltype(r)>(j, j->msgs[j->mcur].p, count, out); });
+  });
+}
+
+int bqa_set_messages(bqa_job* j, int32_t chi, const double* in) {
+  return guarded([&] {
+    require(j && in, "null argument");
+    require(chi == j->chi, "messages must have the tensors' bond dimension");
+    ensure_msgs(j, chi);
+    const size_t count = (size_t)j->M * chi * chi;
+    dispatch(j, [&](auto r) { upload<decltype(r)>(j, j->msgs[j->mcur].p, count, in); });
+  });
+}
+
+int bqa_get_lambdas(bqa_job* j, double* out) {
+  return guarded([&] {
+    require(j && out, "null argument");
+    std::vector<double> h((size_t)j->E * j->lstride);
+    HIPX(hipMemcpyAsync(h.data(), j->lmbd.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, j->stream));
+    HIPX(hipStreamSynchronize(j->stream));
+    for (int e = 0; e < j->E; ++e)
+      for (int k = 0; k < j->chi; ++k) {
+        out[((size_t)e * j->chi + k) * 2] = h[(size_t)e * j->lstride + k];
+        out[((size_t)e * j->chi + k) * 2 + 1] = 0.0;
+      }
+  });
+}
+
+int bqa_set_lambdas(bqa_job* j, int32_t chi, const double* in) {
+  return guarded([&] {
+    require(j && in, "null argument");
+    require(chi == j->chi, "lambdas must have the tensors' bond dimension");
+    j->lmbd.ensure((size_t)j->E * std::max(2, 2 * chi) * sizeof(double), j->dev_bytes, j->stream);
+    std::vector<double> h((size_t)j->E * chi);
+    for (size_t x = 0; x < h.size(); ++x) h[x] = in[2 * x];
+    HIPX(hipStreamSynchronize(j->stream));
+    HIPX(hipMemcpyAsync(j->lmbd.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, j->stream));
+    HIPX(hipStreamSynchronize(j->stream));
+    j->lstride = chi;
+  });
+}
+
+int bqa_timing_enable(bqa_job* j, int32_t on) {
+  return guarded([&] {
+    require(j, "null job");
+    flush_timing(j);
+    j->timing = on != 0;
+    j->t_launches = 0;
+    j->t_ms = 0.0;
+    j->t_bytes = 0.0;
+  });
+}
+
+int bqa_timing_read(bqa_job* j, int64_t* launches, double* total_ms, double* bytes) {
+  return guarded([&] {
+    require(j && launches && total_ms && bytes, "null argument");
+    flush_timing(j);
+    *launches = j->t_launches;
+    *total_ms = j->t_ms;
+    *bytes = j->t_bytes;
+  });
+}
+
+int bqa_copy_calibrate(uint64_t bytes, int32_t repeats, double* ms_per_copy) {
+  return guarded([&] {
+    require(ms_per_copy && bytes >= 16 && repeats > 0, "bad argument");
+    void *a = nullptr, *b = nullptr;
+    HIPX(hipMalloc(&a, bytes));
+    HIPX(hipMalloc(&b, bytes));
+    const long n = (long)(bytes / 16);
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(n, 256, 16384)), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
+    HIPX(hipEventRecord(e0, 0));
+    for (int r = 0; r < repeats; ++r)
+      hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(n, 256, 16384)), dim3(256), 0, 0, (const uint4*)a, (uint4*)b,
+                         n);
+    HIPX(hipEventRecord(e1, 0));
+    HIPX(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+// Shared device helpers: interleaved complex arithmetic, reductions, ZZ split.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bqa {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __host__ __forceinline__ cplx<R> mk(R a, R b) {
+  cplx<R> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
+template <typename R>
+__device__ __host__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) {
+  return mk(a.x + b.x, a.y + b.y);
+}
+template <typename R>
+__device__ __host__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) {
+  return mk(a.x - b.x, a.y - b.y);
+}
+template <typename R>
+__device__ __host__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename R>
+__device__ __host__ __forceinline__ cplx<R> scale(cplx<R> a, R s) {
+  return mk(a.x * s, a.y * s);
+}
+template <typename R>
+__device__ __host__ __forceinline__ cplx<R> conj(cplx<R> a) {
+  return mk(a.x, -a.y);
+}
+// acc += a * b
+template <typename R>
+__device__ __forceinline__ void cfma(cplx<R>& acc, cplx<R> a, cplx<R> b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += conj(a) * b
+template <typename R>
+__device__ __forceinline__ void cfma_cj(cplx<R>& acc, cplx<R> a, cplx<R> b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(-a.y, b.x, acc.y);
+}
+template <typename R>
+__device__ __host__ __forceinline__ R abs2(cplx<R> a) {
+  return a.x * a.x + a.y * a.y;
+}
+template <typename R>
+__device__ __forceinline__ R cabs(cplx<R> a) {
+  return hypot(a.x, a.y);
+}
+// 1 / z
+template <typename R>
+__device__ __forceinline__ cplx<R> cinv(cplx<R> z) {
+  R d = z.x * z.x + z.y * z.y;
+  return mk(z.x / d, -z.y / d);
+}
+
+template <typename To, typename From>
+__device__ __host__ __forceinline__ cplx<To> cast(cplx<From> a) {
+  return mk((To)a.x, (To)a.y);
+}
+
+// Principal square root of a real number as a complex number: sqrt(x) for
+// x >= 0, i*sqrt(-x) for x < 0.
+__device__ __host__ inline cplx<double> csqrt_real(double x) {
+  return x >= 0.0 ? mk(sqrt(x), 0.0) : mk(0.0, sqrt(-x));
+}
+
+// Ket-side split of exp(-i theta ZZ) over one bond:
+// g0 = sqrt(cos theta), g1 = e^{-i pi/4} sqrt(sin theta), so that
+// g0^2 I(x)I + g1^2 Z(x)Z = exp(-i theta ZZ).  Any fixed branch works as long
+// as both bond ends and every use within a step take the same one.
+__device__ __host__ inline void zz_split(double theta, cplx<double>& g0, cplx<double>& g1) {
+  g0 = csqrt_real(cos(theta));
+  cplx<double> s = csqrt_real(sin(theta));
+  const double h = 0.70710678118654752440;
+  g1 = s * mk(h, -h);
+}
+
+// Non-negative double compares like its bit pattern as uint64.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v) {
+  if (!(v >= 0.0)) v = __longlong_as_double(0x7ff8000000000000ULL);  // NaN marker: largest pattern
+  atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+// Block-wide max of a non-negative double (blockDim.x <= 1024, multiple of 64).
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    v = lane < nw ? sh[lane] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  }
+  return v;  // valid in wave 0
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s += sh[w];
+  __syncthreads();
+  return s;  // valid in every thread
+}
+
+__host__ __device__ inline int ipow(int b, int e) {
+  int r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+// BP control block living in device memory.  The sweep kernels read `cur` and
+// `done`; the finish kernel (one thread) updates them after every sweep.
+struct BpStatus {
+  unsigned long long num_bits;  // max |new - old| as double bits
+  unsigned long long den_bits;  // max |new + old|
+  int cur;                      // index of the current message buffer
+  int done;                     // 1 once converged or out of iterations
+  int converged;
+  int iters;                    // sweeps performed
+  int have_best;                // some sweep gave a finite distance
+  int damp_now;                 // the damping kernel must run after this sweep
+  int max_iters;
+  int pad;
+  double bp_eps;
+  double last_dist;
+  double best_dist;
+};
+
+}  // namespace bqa

@@ -1,0 +1,271 @@
+"""ctypes binding of libbqa_hip.so (declarations in include/bqa_hip.h).
+
+The library is built in-tree (``bqa_amd/libbqa_hip.so``) by ``bqa_amd.build``.
+If it is missing or fails to load, importing this module raises ImportError:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+LIB_NAME = "libbqa_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+BQA_OK = 0
+BQA_ERR_INVALID = 1
+BQA_ERR_MEMORY = 2
+BQA_ERR_HIP = 3
+BQA_ERR_NUMERIC = 4
+PREC = {"single": 0, "double": 1}
+
+_i32p = POINTER(c_int32)
+_f64p = POINTER(c_double)
+
+# name -> (restype, argtypes); every function returns an int status
+SIGNATURES = {
+    "bqa_last_error": (c_char_p, []),
+    "bqa_device_info": (c_int, [POINTER(c_int), POINTER(c_int), c_char_p, c_int]),
+    "bqa_job_create": (c_int, [POINTER(c_void_p), c_int32, c_int32, c_int32, _i32p, _i32p, _i32p, _i32p, _i32p,
+                               _i32p, _f64p, _f64p, _i32p, _i32p, _f64p, c_int32, c_int32]),
+    "bqa_job_destroy": (c_int, [c_void_p]),
+    "bqa_job_device_bytes": (c_int, [c_void_p, POINTER(c_uint64)]),
+    "bqa_job_estimate_bytes": (c_int, [c_int32, c_int32, _i32p, _i32p, c_int32, c_int32, POINTER(c_uint64)]),
+    "bqa_init_state": (c_int, [c_void_p]),
+    "bqa_run_bp": (c_int, [c_void_p, c_int32, c_double, c_double, _i32p, _i32p, _f64p]),
+    "bqa_simple_update": (c_int, [c_void_p, c_double, c_double, _i32p, _f64p]),
+    "bqa_apply_local_gates": (c_int, [c_void_p, c_double, c_double]),
+    "bqa_set_symmetric_gauge": (c_int, [c_void_p]),
+    "bqa_run_layer": (c_int, [c_void_p, c_double, c_double, c_double, c_int32, c_double, c_double, _i32p, _f64p,
+                              _i32p, _i32p, _f64p]),
+    "bqa_density": (c_int, [c_void_p, _f64p, _f64p]),
+    "bqa_energy": (c_int, [c_void_p, _f64p, POINTER(c_int8)]),
+    "bqa_bond_dim": (c_int, [c_void_p, _i32p]),
+    "bqa_get_tensors": (c_int, [c_void_p, c_int32, _f64p]),
+    "bqa_set_tensors": (c_int, [c_void_p, c_int32, c_int32, _f64p]),
+    "bqa_get_messages": (c_int, [c_void_p, _f64p]),
+    "bqa_set_messages": (c_int, [c_void_p, c_int32, _f64p]),
+    "bqa_get_lambdas": (c_int, [c_void_p, _f64p]),
+    "bqa_set_lambdas": (c_int, [c_void_p, c_int32, _f64p]),
+    "bqa_timing_enable": (c_int, [c_void_p, c_int32]),
+    "bqa_timing_read": (c_int, [c_void_p, POINTER(c_int64), _f64p, _f64p]),
+    "bqa_copy_calibrate": (c_int, [c_uint64, c_int32, _f64p]),
+}
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+
+def check(rc: int) -> None:
+    """Map a status code to the reference's exception types (src/bqa/state.py:122 for NUMERIC)."""
+    if rc == BQA_OK:
+        return
+    msg = LIB.bqa_last_error().decode(errors="replace")
+    if rc == BQA_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == BQA_ERR_MEMORY:
+        raise MemoryError(msg)
+    if rc == BQA_ERR_NUMERIC:
+        raise AssertionError(msg)
+    raise RuntimeError(msg)
+
+
+def device_info() -> tuple[int, bool, str]:
+    n, ok = c_int(0), c_int(0)
+    name = ctypes.create_string_buffer(64)
+    check(LIB.bqa_device_info(ctypes.byref(n), ctypes.byref(ok), name, 64))
+    return n.value, bool(ok.value), name.value.decode()
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a: np.ndarray, typ):
+    return a.ctypes.data_as(POINTER(typ))
+
+
+def flatten_layouts(ctx) -> dict:
+    """Concatenate a Context's degree-class layouts in class order ([bond][node] per class)."""
+    lays = list(ctx.degree_to_layout.values())
+    out = {
+        "degree": _i32([l.degree for l in lays]),
+        "size": _i32([l.size for l in lays]),
+        "ids": _i32(np.concatenate([l.node_ids for l in lays])),
+        "inp": _i32(np.concatenate([l.input_msgs_position.ravel() for l in lays])),
+        "outp": _i32(np.concatenate([l.output_msgs_position.ravel() for l in lays])),
+        "lp": _i32(np.concatenate([l.lmbds_position.ravel() for l in lays])),
+        "h": _f64(np.concatenate([l.node_ampls for l in lays])),
+        "j": _f64(np.concatenate([l.edge_ampls.ravel() for l in lays])),
+    }
+    return out
+
+
+def estimate_bytes(ctx, max_bond_dim: int, precision: str) -> int:
+    lays = list(ctx.degree_to_layout.values())
+    deg = _i32([l.degree for l in lays])
+    size = _i32([l.size for l in lays])
+    out = c_uint64(0)
+    check(LIB.bqa_job_estimate_bytes(ctx.edges_number, len(lays), _p(deg, c_int32), _p(size, c_int32),
+                                     max_bond_dim, PREC[precision], ctypes.byref(out)))
+    return int(out.value)
+
+
+class Job:
+    """One annealing instance resident on the GPU (a ``bqa_job`` handle)."""
+
+    def __init__(self, ctx, precision: str = "single", max_bond_dim: int | None = None):
+        if precision not in PREC:
+            raise ValueError(f"unknown precision {precision}")
+        self.precision = precision
+        self.nodes_number = ctx.nodes_number
+        self.edges_number = ctx.edges_number
+        self.classes = [(l.degree, l.size) for l in ctx.degree_to_layout.values()]
+        f = flatten_layouts(ctx)
+        self._keep = f
+        lhs, rhs, jv = _i32(ctx.edges.lhs), _i32(ctx.edges.rhs), _f64(ctx.edges.couplings)
+        h = c_void_p()
+        chi_max = int(max_bond_dim if max_bond_dim is not None else ctx.max_bond_dim)
+        check(LIB.bqa_job_create(
+            ctypes.byref(h), ctx.nodes_number, ctx.edges_number, len(self.classes),
+            _p(f["degree"], c_int32), _p(f["size"], c_int32), _p(f["ids"], c_int32), _p(f["inp"], c_int32),
+            _p(f["outp"], c_int32), _p(f["lp"], c_int32), _p(f["h"], c_double), _p(f["j"], c_double),
+            _p(lhs, c_int32), _p(rhs, c_int32), _p(jv, c_double), chi_max, PREC[precision]))
+        self.handle = h
+
+    # lifecycle
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            check(LIB.bqa_job_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_bytes(self) -> int:
+        out = c_uint64(0)
+        check(LIB.bqa_job_device_bytes(self.handle, ctypes.byref(out)))
+        return int(out.value)
+
+    # phases
+    def init_state(self) -> None:
+        check(LIB.bqa_init_state(self.handle))
+
+    def run_bp(self, max_iters: int, bp_eps: float, damping: float) -> tuple[int, bool, float]:
+        s, c, d = c_int32(0), c_int32(0), c_double(0.0)
+        check(LIB.bqa_run_bp(self.handle, int(max_iters), float(bp_eps), float(damping),
+                             ctypes.byref(s), ctypes.byref(c), ctypes.byref(d)))
+        return s.value, bool(c.value), d.value
+
+    def simple_update(self, ztime: float, pinv_eps: float) -> tuple[int, float]:
+        chi, err = c_int32(0), c_double(0.0)
+        check(LIB.bqa_simple_update(self.handle, float(ztime), float(pinv_eps), ctypes.byref(chi),
+                                    ctypes.byref(err)))
+        return chi.value, err.value
+
+    def apply_local_gates(self, ztime: float, xtime: float) -> None:
+        check(LIB.bqa_apply_local_gates(self.handle, float(ztime), float(xtime)))
+
+    def set_symmetric_gauge(self) -> None:
+        check(LIB.bqa_set_symmetric_gauge(self.handle))
+
+    def run_layer(self, xtime: float, ztime: float, pinv_eps: float, max_iters: int, bp_eps: float,
+                  damping: float) -> dict:
+        chi, err = c_int32(0), c_double(0.0)
+        s, c, d = c_int32(0), c_int32(0), c_double(0.0)
+        check(LIB.bqa_run_layer(self.handle, float(xtime), float(ztime), float(pinv_eps), int(max_iters),
+                                float(bp_eps), float(damping), ctypes.byref(chi), ctypes.byref(err),
+                                ctypes.byref(s), ctypes.byref(c), ctypes.byref(d)))
+        return {"bond_dim": chi.value, "trunc_err": err.value, "sweeps": s.value,
+                "converged": bool(c.value), "dist": d.value}
+
+    def density(self) -> tuple[np.ndarray, np.ndarray]:
+        rho = np.empty((self.nodes_number, 2, 2), dtype=np.complex128)
+        bloch = np.empty((self.nodes_number, 3), dtype=np.float64)
+        check(LIB.bqa_density(self.handle, rho.ctypes.data_as(_f64p), bloch.ctypes.data_as(_f64p)))
+        return rho, bloch
+
+    def energy(self) -> tuple[float, np.ndarray]:
+        e = c_double(0.0)
+        spins = np.empty(self.nodes_number, dtype=np.int8)
+        check(LIB.bqa_energy(self.handle, ctypes.byref(e), spins.ctypes.data_as(POINTER(c_int8))))
+        return e.value, spins
+
+    @property
+    def bond_dim(self) -> int:
+        out = c_int32(0)
+        check(LIB.bqa_bond_dim(self.handle, ctypes.byref(out)))
+        return out.value
+
+    # state transfer (complex128 host arrays)
+    def get_tensors(self, class_index: int) -> np.ndarray:
+        d, n = self.classes[class_index]
+        chi = self.bond_dim
+        out = np.empty((n, 2) + (chi,) * d, dtype=np.complex128)
+        if out.size:
+            check(LIB.bqa_get_tensors(self.handle, class_index, out.ctypes.data_as(_f64p)))
+        return out
+
+    def set_tensors(self, class_index: int, tensors: np.ndarray) -> None:
+        d, n = self.classes[class_index]
+        t = np.ascontiguousarray(tensors, dtype=np.complex128)
+        chi = t.shape[2] if d > 0 else self.bond_dim
+        if t.shape != (n, 2) + (chi,) * d:
+            raise ValueError(f"tensor shape {t.shape} does not match class ({d}, {n})")
+        check(LIB.bqa_set_tensors(self.handle, class_index, chi, t.ctypes.data_as(_f64p)))
+
+    def get_messages(self) -> np.ndarray:
+        chi = self.bond_dim
+        out = np.empty((self.edges_number, chi, chi), dtype=np.complex128)
+        check(LIB.bqa_get_messages(self.handle, out.ctypes.data_as(_f64p)))
+        return out
+
+    def set_messages(self, msgs: np.ndarray) -> None:
+        m = np.ascontiguousarray(msgs, dtype=np.complex128)
+        check(LIB.bqa_set_messages(self.handle, m.shape[1], m.ctypes.data_as(_f64p)))
+
+    def get_lambdas(self) -> np.ndarray:
+        chi = self.bond_dim
+        out = np.empty((self.edges_number // 2, chi), dtype=np.complex128)
+        check(LIB.bqa_get_lambdas(self.handle, out.ctypes.data_as(_f64p)))
+        return out
+
+    def set_lambdas(self, lmbds: np.ndarray) -> None:
+        l = np.ascontiguousarray(lmbds, dtype=np.complex128)
+        check(LIB.bqa_set_lambdas(self.handle, l.shape[1], l.ctypes.data_as(_f64p)))
+
+    # measurement helpers
+    def timing_enable(self, on: bool) -> None:
+        check(LIB.bqa_timing_enable(self.handle, 1 if on else 0))
+
+    def timing_read(self) -> tuple[int, float, float]:
+        n, ms, b = c_int64(0), c_double(0.0), c_double(0.0)
+        check(LIB.bqa_timing_read(self.handle, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+        return n.value, ms.value, b.value
+
+
+def copy_calibrate(nbytes: int, repeats: int = 20) -> float:
+    ms = c_double(0.0)
+    check(LIB.bqa_copy_calibrate(int(nbytes), int(repeats), ctypes.byref(ms)))
+    return ms.value

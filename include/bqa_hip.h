@@ -1,0 +1,158 @@
+/*
+ * bqa_hip.h -- C ABI of libbqa_hip.so, the MI355X (gfx950) backend for
+ * belief-propagation tensor-network quantum-annealing emulation.
+ *
+ * One job = one annealing instance resident on one GPU: degree-class node
+ * tensors, two message buffers, Vidal-gauge lambdas and the simple-update
+ * workspaces, all in device memory owned by the library.  The host drives
+ * the reference's state machine (src/bqa/state.py) through phase entry points;
+ * each phase replaces a group of the reference's batched backend calls
+ * (src/bqa/backends.py) and runs as hand-written HIP kernels.
+ *
+ * Conventions
+ *   - every function returns 0 on success or a BQA_ERR_* code; the message of
+ *     the last failure on the calling thread is bqa_last_error();
+ *   - arrays are plain host pointers; complex numbers are interleaved
+ *     (re, im) doubles; layout arrays are int32 laid out [bond][node];
+ *   - message position p is the reference's position (config_canonicalization.py:189-205):
+ *     forward edges 0..E-1 in input order, reversed edges E..2E-1;
+ *   - a job is not thread-safe; distinct jobs may run on distinct threads.
+ */
+#ifndef BQA_HIP_H
+#define BQA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BQA_OK 0
+#define BQA_ERR_INVALID 1      /* bad argument or shape   -> ValueError   */
+#define BQA_ERR_MEMORY 2       /* device allocation failed -> MemoryError  */
+#define BQA_ERR_HIP 3          /* HIP runtime error        -> RuntimeError */
+#define BQA_ERR_NUMERIC 4      /* BP produced no finite distance (state.py:122) */
+
+#define BQA_PREC_SINGLE 0      /* complex64 tensors and messages  */
+#define BQA_PREC_DOUBLE 1      /* complex128 tensors and messages */
+
+typedef struct bqa_job bqa_job;
+
+/* Thread-local text of the last error (never NULL). */
+const char* bqa_last_error(void);
+
+/* Number of visible gfx950 devices (0 when none); *arch_ok=1 when device 0 is gfx950. */
+int bqa_device_info(int* device_count, int* arch_ok, char* name_out, int name_len);
+
+/*
+ * Create a job from a compiled degree-grouped layout
+ * (replaces the backend.make_from_list uploads of
+ *  src/bqa/config/config_canonicalization.py:109-126).
+ *
+ *   nodes_number, edges_number  -- qubits n and directed messages 2E
+ *   classes                     -- number of degree classes
+ *   class_degree[c], class_size[c]
+ *   node_ids      -- concat over classes of (n_c) ids
+ *   input_pos, output_pos, lmbd_pos -- concat over classes of (d_c * n_c), [bond][node]
+ *   node_ampls    -- concat over classes of (n_c) fields h
+ *   edge_ampls    -- concat over classes of (d_c * n_c) couplings J, [bond][node]
+ *   fwd_lhs, fwd_rhs, fwd_j -- the E forward edges (energy readout)
+ *   max_bond_dim  -- chi_max (device buffers are sized for it)
+ *   precision     -- BQA_PREC_SINGLE or BQA_PREC_DOUBLE
+ */
+int bqa_job_create(bqa_job** job, int32_t nodes_number, int32_t edges_number, int32_t classes,
+                   const int32_t* class_degree, const int32_t* class_size,
+                   const int32_t* node_ids, const int32_t* input_pos, const int32_t* output_pos,
+                   const int32_t* lmbd_pos, const double* node_ampls, const double* edge_ampls,
+                   const int32_t* fwd_lhs, const int32_t* fwd_rhs, const double* fwd_j,
+                   int32_t max_bond_dim, int32_t precision);
+
+int bqa_job_destroy(bqa_job* job);
+
+/* Device bytes held by the job (admission accounting). */
+int bqa_job_device_bytes(const bqa_job* job, uint64_t* bytes);
+
+/* Bytes a job of this shape would hold, without creating it. */
+int bqa_job_estimate_bytes(int32_t edges_number, int32_t classes, const int32_t* class_degree,
+                           const int32_t* class_size, int32_t max_bond_dim, int32_t precision,
+                           uint64_t* bytes);
+
+/* |-> on every qubit, bond dimension 1, lambda = 1, messages [[1]]
+ * (src/bqa/state.py:36-74). */
+int bqa_init_state(bqa_job* job);
+
+/*
+ * Jacobi BP to convergence (src/bqa/state.py:97-124): dist = max|new-old| /
+ * max|new+old| over all messages; stop before applying the update when
+ * dist < bp_eps; otherwise msgs <- damping*msgs + (1-damping)*new; after
+ * max_iters sweeps the messages are the last undamped iterate.
+ * Outputs: *sweeps (sweeps run), *converged (0/1), *last_dist.
+ * Returns BQA_ERR_NUMERIC when no sweep gave a finite distance.
+ */
+int bqa_run_bp(bqa_job* job, int32_t max_iters, double bp_eps, double damping,
+               int32_t* sweeps, int32_t* converged, double* last_dist);
+
+/*
+ * ZZ layer with truncation (src/bqa/state.py:230-247): extended messages,
+ * FP64 canonicalisers (SVD via one-sided Jacobi), the global bond dimension
+ * chi' = min(chi_max, #{k: max_e lambda_e[k] >= pinv_eps}) and absorption of
+ * the canonicalisers into the node tensors.  No gates, no gauge.
+ * Outputs: *new_bond_dim, *trunc_err (norm of the discarded max-lambdas).
+ */
+int bqa_simple_update(bqa_job* job, double ztime, double pinv_eps,
+                      int32_t* new_bond_dim, double* trunc_err);
+
+/* Local Z layer exp(-i ztime h Z) then X layer exp(-i xtime X) on every
+ * qubit (src/bqa/state.py:142-156).  Pass 0 to skip either. */
+int bqa_apply_local_gates(bqa_job* job, double ztime, double xtime);
+
+/* Symmetric gauge (src/bqa/state.py:219-227): T <- T prod sqrt(lambda),
+ * L2-normalised per node; messages <- diag(lambda) / sum(lambda). */
+int bqa_set_symmetric_gauge(bqa_job* job);
+
+/*
+ * One Trotter step, fused (src/bqa/state.py:315-321): simple update, Z and X
+ * layers, symmetric gauge and BP in one pass of kernels; outputs as above.
+ */
+int bqa_run_layer(bqa_job* job, double xtime, double ztime, double pinv_eps,
+                  int32_t max_iters, double bp_eps, double damping,
+                  int32_t* new_bond_dim, double* trunc_err,
+                  int32_t* sweeps, int32_t* converged, double* last_dist);
+
+/*
+ * Per-qubit reduced density matrices and Bloch vectors
+ * (src/bqa/state.py:77-94, backends.py:440-448, utils.py:23-27).
+ *   rho_out   -- n * 2 * 2 complex (n*8 doubles), indexed by qubit id; may be NULL
+ *   bloch_out -- n * 3 doubles (x, y, z); may be NULL
+ */
+int bqa_density(bqa_job* job, double* rho_out, double* bloch_out);
+
+/* FP64 Ising energy of the rounded spins s_i = +1 iff <Z_i> > 0 of the last
+ * bqa_density call: sum_i h_i s_i + sum_(ij) J_ij s_i s_j; spins_out (n int8) may be NULL. */
+int bqa_energy(bqa_job* job, double* energy, int8_t* spins_out);
+
+/* Current bond dimension. */
+int bqa_bond_dim(const bqa_job* job, int32_t* chi);
+
+/* State transfer (testing and checkpoint): complex128 host arrays.
+ * tensors of class c: (n_c, 2, chi^d_c); messages: (2E, chi, chi); lambdas: (E, chi). */
+int bqa_get_tensors(bqa_job* job, int32_t class_index, double* out);
+int bqa_set_tensors(bqa_job* job, int32_t class_index, int32_t chi, const double* in);
+int bqa_get_messages(bqa_job* job, double* out);
+int bqa_set_messages(bqa_job* job, int32_t chi, const double* in);
+int bqa_get_lambdas(bqa_job* job, double* out);
+int bqa_set_lambdas(bqa_job* job, int32_t chi, const double* in);
+
+/* Kernel timing of the BP sweep kernels, recorded with HIP events on the job's
+ * stream while enabled: launches, total milliseconds, algorithmic bytes moved. */
+int bqa_timing_enable(bqa_job* job, int32_t on);
+int bqa_timing_read(bqa_job* job, int64_t* launches, double* total_ms, double* bytes);
+
+/* Plain device-to-device copy of `bytes` bytes (HBM counter calibration). */
+int bqa_copy_calibrate(uint64_t bytes, int32_t repeats, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BQA_HIP_H */

@@ -1,0 +1,215 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle and the reference's golden vectors.
+
+Tolerances: FP64 GPU path 1e-8 on Bloch vectors (same trajectory, same bond
+dimensions, same BP sweep counts); FP32 path 1e-4 on Bloch vectors.  Phase
+checks compare gauge-fixed quantities only (messages for given tensors,
+tensors after local gates and gauge, densities, lambdas).
+"""
+import numpy as np
+import pytest
+
+from bqa_amd import instances
+from bqa_amd.config import config_to_context
+from bqa_amd.native import Job
+from bqa_amd.state import _initialize_state, get_bloch_vectors, run_layer
+from conftest import golden_names, load_golden
+from oracle import bqa_oracle as O
+from treeutil import SMALL_TREE_CONFIG, branch_message, per_node, random_tree_tensors
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"double": 1e-8, "single": 1e-4}
+
+
+def run_product(cfg, precision, fused=True):
+    ctx = config_to_context(dict(cfg, backend="mi355x", precision=precision))
+    st = _initialize_state(ctx)
+    try:
+        bloch = None
+        for ins in ctx.instructions:
+            if isinstance(ins, dict):
+                run_layer(ctx, ins["xtime"], ins["ztime"], st, fused=fused)
+            elif ins == "get_bloch_vectors":
+                bloch = get_bloch_vectors(ctx, st)
+        lm = np.sort(st.backend.job.get_lambdas().real, axis=1)[:, ::-1]
+        return bloch, st, lm
+    finally:
+        st.backend.close()
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_fp64(name):
+    cfg, want = load_golden(name)
+    print(f"[gpu] golden fp64 {name}", flush=True)
+    bloch, st, lm = run_product(cfg, "double")
+    assert np.abs(bloch - want["bloch"]).max() < TOL["double"]
+    assert st.chis == want["chis"].tolist()
+    assert st.bp_sweeps == want["sweeps"].tolist()
+    assert np.abs(lm - want["lmbds"]).max() < 1e-8
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_fp32(name):
+    cfg, want = load_golden(name)
+    print(f"[gpu] golden fp32 {name}", flush=True)
+    bloch, st, _ = run_product(cfg, "single")
+    err = np.abs(bloch - want["bloch"]).max()
+    print(f"[gpu]   fp32 max |bloch error| = {err:.3e}", flush=True)
+    assert err < TOL["single"]
+    assert st.chis == want["chis"].tolist()
+
+
+def test_fused_equals_phase_by_phase():
+    cfg, _ = load_golden("grid6x6_chi8")
+    a, _, _ = run_product(cfg, "double", fused=True)
+    b, _, _ = run_product(cfg, "double", fused=False)
+    assert np.abs(a - b).max() < 1e-12
+
+
+def _upload(job, ctx_o, st_o):
+    for ci, d in enumerate(st_o.tensors):
+        job.set_tensors(ci, st_o.tensors[d])
+    job.set_lambdas(st_o.lmbds)
+    job.set_messages(st_o.msgs)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("chi", [1, 2, 3])
+def test_bp_tree_closed_form(precision, chi):
+    """BP messages on a tree equal the brute-force branch environments."""
+    ctx_o = O.make_context(SMALL_TREE_CONFIG)
+    st_o = O.init_state(ctx_o)
+    st_o.tensors = random_tree_tensors(ctx_o, chi, np.random.default_rng(100 + chi))
+    st_o.chi = chi
+    st_o.lmbds = np.ones((ctx_o.lmbds_number, chi), dtype=O.DTYPE)
+    st_o.msgs = O.msgs_from_lmbds(ctx_o, st_o.lmbds)
+    ctx = config_to_context(dict(SMALL_TREE_CONFIG, max_bond_dim=4))
+    job = Job(ctx, precision)
+    try:
+        job.init_state()
+        _upload(job, ctx_o, st_o)
+        sweeps, conv, dist = job.run_bp(100, 1e-12 if precision == "double" else 1e-6, 0.0)
+        msgs = job.get_messages()
+    finally:
+        job.close()
+    tens = per_node(ctx_o, st_o.tensors)
+    tol = 1e-10 if precision == "double" else 2e-5
+    for (u, v), pos in ctx_o.edge_to_msg_pos.items():
+        assert np.abs(msgs[pos] - branch_message(ctx_o.graph, tens, u, v)).max() < tol
+
+
+def _mid_anneal(name, steps):
+    cfg, _ = load_golden(name)
+    ctx_o = O.make_context(cfg)
+    st_o = O.init_state(ctx_o)
+    for ins in [i for i in ctx_o.instructions if isinstance(i, dict)][:steps]:
+        O.run_layer(ctx_o, st_o, ins["xtime"], ins["ztime"])
+    return cfg, ctx_o, st_o
+
+
+@pytest.mark.parametrize("name,steps", [("rr100_fields_chi4", 6), ("grid6x6_chi8", 5), ("heavyhex_chi6", 6)])
+def test_phases_against_oracle(name, steps):
+    """Upload an oracle mid-anneal state, run each phase on the GPU, compare (FP64)."""
+    cfg, ctx_o, st_o = _mid_anneal(name, steps)
+    ctx = config_to_context(dict(cfg, backend="mi355x"))
+    job = Job(ctx, "double")
+    try:
+        job.init_state()
+        _upload(job, ctx_o, st_o)
+        # density readout
+        rho, bloch = job.density()
+        assert np.abs(rho - O.get_density_matrices(ctx_o, st_o)).max() < 1e-10
+        # BP from the current messages: same tensors -> same messages
+        o2 = O.State(tensors={d: t.copy() for d, t in st_o.tensors.items()}, msgs=st_o.msgs.copy(),
+                     lmbds=st_o.lmbds.copy(), chi=st_o.chi)
+        O.run_bp(ctx_o, o2)
+        sweeps, conv, dist = job.run_bp(ctx_o.max_bp_iters, ctx_o.bp_eps, ctx_o.damping)
+        assert sweeps == o2.bp_iters[-1]
+        assert np.abs(job.get_messages() - o2.msgs).max() < 1e-10
+        # simple update: same global chi' and lambdas (sorted, gauge invariant)
+        ins = [i for i in ctx_o.instructions if isinstance(i, dict)][steps]
+        O.simple_update(ctx_o, o2, ins["ztime"])
+        chi, err = job.simple_update(ins["ztime"], ctx_o.pinv_eps)
+        assert chi == o2.chi
+        assert abs(err - o2.trunc_errs[-1]) < 1e-10
+        lm = np.sort(job.get_lambdas().real, axis=1)
+        assert np.abs(lm - np.sort(o2.lmbds.real, axis=1)).max() < 1e-10
+        # local gates and gauge act on the (gauge-dependent) tensors: compare after
+        # re-uploading the oracle's absorbed tensors
+        for ci, d in enumerate(o2.tensors):
+            job.set_tensors(ci, o2.tensors[d])
+        job.set_lambdas(o2.lmbds)
+        O.apply_z_layer(ctx_o, o2, ins["ztime"])
+        O.apply_x_layer(ctx_o, o2, ins["xtime"])
+        job.apply_local_gates(ins["ztime"], ins["xtime"])
+        for ci, d in enumerate(o2.tensors):
+            assert np.abs(job.get_tensors(ci) - o2.tensors[d]).max() < 1e-12
+        O.set_symmetric_gauge(ctx_o, o2)
+        job.set_symmetric_gauge()
+        for ci, d in enumerate(o2.tensors):
+            assert np.abs(job.get_tensors(ci) - o2.tensors[d]).max() < 1e-12
+        assert np.abs(job.get_messages() - o2.msgs).max() < 1e-14
+    finally:
+        job.close()
+
+
+def test_isolated_qubits_and_degree_one():
+    """Degree-0 (isolated) and degree-1 classes (the reference asserts on degree 0 in readout)."""
+    nodes, edges = instances.heavy_hex_gaussian(2, 3, seed=8)
+    n = max(max(e) for e in edges) + 1
+    nodes = dict(nodes)
+    nodes[n] = 0.7          # isolated qubits
+    nodes[n + 1] = -0.2
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": 4, "bp_eps": 1e-10, "max_bp_iter_number": 100,
+           "schedule": instances.anneal_schedule(1.0, 10)}
+    want = np.array(O.run_qa(cfg)[0][1])
+    got, _, _ = run_product(cfg, "double")
+    assert np.abs(got - want).max() < 1e-9
+    # an isolated qubit just precesses: |bloch| = 1
+    assert abs(np.linalg.norm(got[n]) - 1.0) < 1e-12
+
+
+def test_bp_iteration_limit_and_damping_rules():
+    cfg, want = load_golden("rr100_nonconverged")
+    _, st, _ = run_product(cfg, "double")
+    assert st.bp_sweeps == want["sweeps"].tolist()
+    cfg, want = load_golden("rr100_damped")
+    _, st, _ = run_product(cfg, "double")
+    assert st.bp_sweeps == want["sweeps"].tolist()
+
+
+def test_error_paths():
+    nodes, edges = instances.regular_pm1(10, 3, seed=1)
+    ctx = config_to_context({"nodes": nodes, "edges": edges})
+    job = Job(ctx, "single")
+    try:
+        job.init_state()
+        with pytest.raises(AssertionError):
+            job.run_bp(0, 1e-6, 0.0)       # state.py:122: no sweep, no messages
+        with pytest.raises(ValueError):
+            job.run_bp(10, 1e-6, 1.5)      # damping outside [0, 1]
+        with pytest.raises(ValueError):
+            job.set_messages(np.zeros((ctx.edges_number, 3, 3)))
+    finally:
+        job.close()
+    with pytest.raises(ValueError):
+        Job(ctx, "single", max_bond_dim=64)
+
+
+def test_energy_of_rounded_spins():
+    nodes, edges = instances.regular_uniform(30, 3, seed=4, field_sigma=0.5)
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": 4, "schedule": instances.anneal_schedule(3.0, 15)}
+    ctx = config_to_context(dict(cfg, precision="double"))
+    st = _initialize_state(ctx)
+    try:
+        for ins in ctx.instructions:
+            if isinstance(ins, dict):
+                run_layer(ctx, ins["xtime"], ins["ztime"], st)
+        bloch = get_bloch_vectors(ctx, st)
+        e, spins = st.backend.energy()
+    finally:
+        st.backend.close()
+    s = np.where(bloch[:, 2] > 0, 1, -1)
+    assert np.array_equal(spins, s)
+    want = sum(h * s[v] for v, h in nodes.items()) + sum(j * s[a] * s[b] for (a, b), j in edges.items())
+    assert abs(e - want) < 1e-9 * max(1.0, abs(want))
