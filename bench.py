@@ -1,0 +1,244 @@
+"""Benchmark: qubit*Trotter-steps/s of the 100k-qubit random 3-regular QUBO anneal on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 8] [--no-accuracy] [--no-cpu]
+
+Workload: random 3-regular graph on 100,000 qubits (configuration model,
+seed 3), QUBO weights uniform integers in [-10, 10] on edges and diagonal,
+mapped to Ising form; linear anneal T = 20 over 200 Trotter steps; bond
+dimension chi = 8 (headline) and chi = 4 (reported beside it); BP with the
+reference defaults bp_eps = 1e-6, at most 75 sweeps, pinv_eps = 1e-6.
+A "step" is one Trotter step (simple update + local gates + gauge + BP to
+convergence).  W untimed warmup steps (the chi ramp) precede K timed steps;
+the schedule has max(200, W + K) steps.  Inputs are resident in HBM before
+the timed region starts.
+
+The FP32 path is timed.  Accuracy: the same anneal in FP64 on the GPU
+(untimed) gives the per-qubit <Z> error, rounded-spin disagreements and the
+energy error.  cpu_baseline: the FP64 NumPy oracle on a bounded sample of the
+same workload (1,000-qubit instance from the same generator, three Trotter
+steps after the chi ramp, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "qubit·steps/s, 100k-qubit 3-reg anneal @⟨Z⟩ err≤1e-4; jobs/h + p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+N_QUBITS = 100_000
+TOTAL_TIME = 20.0
+BASE_STEPS = 200
+SEED = 3
+BP_EPS = 1e-6
+BP_ITERS = 75
+PINV_EPS = 1e-6
+
+
+def make_config(n, chi, steps, precision):
+    from bqa_amd import instances
+    nodes, edges, _ = instances.regular_qubo(n, 3, seed=SEED, qmax=10)
+    return {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "bp_eps": BP_EPS,
+            "max_bp_iter_number": BP_ITERS, "pinv_eps": PINV_EPS, "damping": 0.0,
+            "schedule": instances.anneal_schedule(TOTAL_TIME, steps), "backend": "mi355x",
+            "precision": precision}
+
+
+def run_gpu(ctx, precision, warmup, steps, timed):
+    from bqa_amd.native import Job
+    job = Job(ctx, precision)
+    try:
+        job.init_state()
+        layers = [i for i in ctx.instructions if isinstance(i, dict)]
+        stats = []
+
+        def step(ins):
+            return job.run_layer(ins["xtime"], ins["ztime"], ctx.pinv_eps, ctx.max_bp_iters_number,
+                                 ctx.bp_eps, ctx.damping)
+
+        for ins in layers[:warmup]:
+            stats.append(step(ins))
+        job.density()                         # synchronises the stream
+        if timed:
+            job.timing_enable(True)
+        t0 = time.perf_counter()
+        for ins in layers[warmup:warmup + steps]:
+            stats.append(step(ins))
+        t1 = time.perf_counter()
+        kern = job.timing_read() if timed else (0, 0.0, 0.0)
+        job.timing_enable(False)
+        rho, bloch = job.density()
+        energy, spins = job.energy()
+        return {"seconds": t1 - t0, "stats": stats, "bloch": bloch, "energy": energy, "spins": spins,
+                "kernel": kern, "device_bytes": job.device_bytes()}
+    finally:
+        job.close()
+
+
+def load_traffic():
+    """HBM bytes per BP-sweep launch measured with rocprofv3 counters (profiles/)."""
+    path = os.path.join(ROOT, "profiles", "bp_sweep_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(chi):
+    """FP64 NumPy oracle, one thread, 1,000-qubit sample of the same workload."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:              # pragma: no cover
+        threadpool_limits = None
+    from oracle import bqa_oracle as O
+    n = 1000
+    cfg = make_config(n, chi, BASE_STEPS, "double")
+    ctx = O.make_context(cfg)
+    st = O.init_state(ctx)
+    layers = [i for i in ctx.instructions if isinstance(i, dict)]
+    ctl = threadpool_limits(limits=1) if threadpool_limits else None
+    try:
+        ramp = 0
+        while ramp < 80 and st.chi < chi:        # untimed: run until the bond dimension reaches chi
+            ins = layers[ramp]
+            O.run_layer(ctx, st, ins["xtime"], ins["ztime"])
+            ramp += 1
+        s_timed = 3
+        t0 = time.perf_counter()
+        for ins in layers[ramp:ramp + s_timed]:
+            O.run_layer(ctx, st, ins["xtime"], ins["ztime"])
+        dt = time.perf_counter() - t0
+    finally:
+        if ctl is not None:
+            ctl.unregister() if hasattr(ctl, "unregister") else None
+    sweeps = st.bp_iters[ramp:ramp + s_timed]
+    return {"value": n * s_timed / dt, "unit": "qubit·steps/s", "cores": 1, "kind": "port",
+            "sample": (f"FP64 NumPy oracle, {n}-qubit random 3-regular QUBO (same generator), chi={chi}, "
+                       f"Trotter steps {ramp + 1}-{ramp + s_timed} of {BASE_STEPS}, the first at bond dimension {chi}, "
+                       f"BP sweeps per step {sweeps}, {dt:.1f} s"),
+            "seconds_per_step": dt / s_timed, "host_cpus": len(os.sched_getaffinity(0))}
+
+
+def accuracy(fast, ref):
+    z32, z64 = fast["bloch"][:, 2], ref["bloch"][:, 2]
+    dz = np.abs(z32 - z64)
+    decided = np.abs(z64) >= 1e-3
+    spin_mismatch = int(np.sum((fast["spins"] != ref["spins"]) & decided))
+    e_rel = abs(fast["energy"] - ref["energy"]) / max(1.0, abs(ref["energy"]))
+    return {"max_abs_dz": float(dz.max()), "mean_abs_dz": float(dz.mean()),
+            "max_abs_dbloch": float(np.abs(fast["bloch"] - ref["bloch"]).max()),
+            "spin_mismatches_decided": spin_mismatch, "energy_fp32": fast["energy"],
+            "energy_fp64": ref["energy"], "energy_rel_err": e_rel}
+
+
+def summarize(run, n, steps):
+    stats = run["stats"][-steps:] if steps else []
+    sweeps = [s["sweeps"] for s in stats]
+    launches, ms, bytes_ = run["kernel"]
+    return {"qubit_steps_per_s": n * steps / run["seconds"], "ms_per_step": 1e3 * run["seconds"] / steps,
+            "bp_sweeps_total": int(sum(sweeps)), "bp_sweeps_per_step_mean": float(np.mean(sweeps)),
+            "bond_dim_final": stats[-1]["bond_dim"] if stats else None, "bp_kernel_launches": launches,
+            "bp_kernel_ms": ms, "bp_kernel_bytes": bytes_, "device_bytes": run["device_bytes"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=195)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--chi", type=int, default=8)
+    ap.add_argument("--qubits", type=int, default=N_QUBITS)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-chi4", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(independent jobs would shard one per GPU with no communication)", file=sys.stderr)
+        sys.exit(2)
+    from bqa_amd.config import config_to_context
+    from bqa_amd.native import device_info
+    n_dev, arch_ok, arch = device_info()
+    if n_dev < 1:
+        raise RuntimeError("no HIP device")
+    n = args.qubits
+    total_steps = max(BASE_STEPS, args.warmup + args.steps)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+
+    def one(chi, want_acc):
+        ctx32 = config_to_context(make_config(n, chi, total_steps, "single"))
+        log(f"[bench] chi={chi} fp32: {args.warmup} warmup + {args.steps} timed Trotter steps, n={n}")
+        fast = run_gpu(ctx32, "single", args.warmup, args.steps, timed=True)
+        s = summarize(fast, n, args.steps)
+        log(f"[bench] chi={chi} fp32: {s['qubit_steps_per_s']:.4g} qubit*steps/s, {s['ms_per_step']:.2f} ms/step, "
+            f"BP sweeps/step {s['bp_sweeps_per_step_mean']:.1f}")
+        out = {"summary": s, "run": fast, "ctx": ctx32}
+        if want_acc:
+            ctx64 = config_to_context(make_config(n, chi, total_steps, "double"))
+            log(f"[bench] chi={chi} fp64 reference run (untimed)")
+            ref = run_gpu(ctx64, "double", args.warmup, args.steps, timed=False)
+            out["accuracy"] = accuracy(fast, ref)
+            log(f"[bench] chi={chi} accuracy {out['accuracy']}")
+        return out
+
+    head = one(args.chi, not args.no_accuracy)
+    s = head["summary"]
+    launches, ms, bytes_ = head["run"]["kernel"]
+    achieved = (bytes_ / 1e9) / (ms / 1e3) if ms > 0 else 0.0
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": (traffic or {}).get("hbm_bytes_per_launch") if traffic else None,
+                "kernel": "BP sweep (one launch per degree class per sweep)",
+                "launches": launches, "avg_launch_us": round(1e3 * ms / max(1, launches), 3),
+                "algorithmic_bytes_per_launch": bytes_ / max(1, launches)}
+    extra = {}
+    if not args.no_chi4 and args.chi != 4:
+        c4 = one(4, not args.no_accuracy)
+        extra["chi4"] = dict(c4["summary"], accuracy=c4.get("accuracy"))
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (FP64 NumPy oracle)")
+        cpu = cpu_baseline(args.chi)
+        log(f"[bench] cpu baseline {cpu['value']:.4g} qubit*steps/s")
+    E = head["ctx"].edges_number
+    result = {
+        "metric": METRIC,
+        "value": round(s["qubit_steps_per_s"], 2),
+        "unit": "qubit·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}-qubit random 3-regular QUBO anneal (ints in [-10,10]), chi={args.chi}, "
+                               f"T={TOTAL_TIME:g}, {total_steps} Trotter steps",
+                   "qubits": n, "edges": E // 2, "chi": args.chi, "bp_eps": BP_EPS,
+                   "max_bp_iter_number": BP_ITERS, "pinv_eps": PINV_EPS, "precision": "fp32 tensors/messages, "
+                   "fp64 SVD/lambdas/energy", "device": arch},
+        "message_updates_per_s": round(E * s["bp_sweeps_total"] / max(1e-12, s["ms_per_step"] * args.steps / 1e3), 1),
+        "bp_sweeps_per_step": s["bp_sweeps_per_step_mean"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "accuracy": head.get("accuracy"),
+    }
+    result.update(extra)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

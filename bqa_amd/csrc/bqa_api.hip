@@ -409,7 +409,8 @@ void launch_bp_class(bqa_job* j, ClassDev& c) {
 template <typename R>
 void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* sweeps, int* converged,
                  double* last_dist) {
-  require(max_iters > 0, "max_bp_iter_number must be positive: BP needs at least one sweep (state.py:122)");
+  if (max_iters <= 0)  // the reference asserts here (src/bqa/state.py:122)
+    throw BqaError(BQA_ERR_NUMERIC, "max_bp_iter_number must be positive: BP ran no sweep");
   BpStatus* st = reinterpret_cast<BpStatus*>(j->status.p);
   hipLaunchKernelGGL(k_bp_reset, dim3(1), dim3(1), 0, j->stream, st, j->mcur, max_iters, bp_eps);
   LAUNCH_CHECK();

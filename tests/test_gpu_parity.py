@@ -142,8 +142,9 @@ def test_phases_against_oracle(name, steps):
         O.apply_z_layer(ctx_o, o2, ins["ztime"])
         O.apply_x_layer(ctx_o, o2, ins["xtime"])
         job.apply_local_gates(ins["ztime"], ins["xtime"])
-        for ci, d in enumerate(o2.tensors):
-            assert np.abs(job.get_tensors(ci) - o2.tensors[d]).max() < 1e-12
+        for ci, d in enumerate(o2.tensors):     # absorbed tensors are unnormalised: relative check
+            scale = np.abs(o2.tensors[d]).max()
+            assert np.abs(job.get_tensors(ci) - o2.tensors[d]).max() < 1e-13 * scale
         O.set_symmetric_gauge(ctx_o, o2)
         job.set_symmetric_gauge()
         for ci, d in enumerate(o2.tensors):
