@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <new>
@@ -20,6 +21,7 @@
 #include "../../include/bqa_hip.h"
 #include "common.hpp"
 #include "kernels_canon.hpp"
+#include "kernels_canon_reg.hpp"
 #include "kernels_fast.hpp"
 #include "kernels_node.hpp"
 
@@ -147,6 +149,7 @@ struct bqa_job {
   int64_t t_launches = 0;
   double t_ms = 0.0, t_bytes = 0.0;
   std::vector<double> pending_bytes;
+  bool generic_only = false;  // BQA_GENERIC_KERNELS=1: run the generic kernels for every shape
 
   bool single() const { return prec == BQA_PREC_SINGLE; }
 };
@@ -459,6 +462,34 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
   if (!h.have_best) throw BqaError(BQA_ERR_NUMERIC, "BP produced no finite message distance");
 }
 
+template <typename R, int N>
+void launch_canon_reg(bqa_job* j, int ccols, double pinv_eps, double deps) {
+  using S = RegCanonShape<N>;
+  j->cws.ensure((size_t)j->E * S::WS * sizeof(double), j->dev_bytes, j->stream);
+  const int grid = (j->E + S::EPW - 1) / S::EPW;
+  hipLaunchKernelGGL((k_canon_reg<R, N>), dim3(grid), dim3(64), 0, j->stream,
+                     reinterpret_cast<const cplx<R>*>(j->ext.p), j->E, ccols, pinv_eps, deps,
+                     reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p),
+                     reinterpret_cast<double*>(j->cws.p));
+  LAUNCH_CHECK();
+}
+
+// Register-resident canonicalisers for chi <= 16; false for larger chi.
+template <typename R>
+bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double deps) {
+  switch (chi) {
+    case 1: launch_canon_reg<R, 2>(j, ccols, pinv_eps, deps); return true;
+    case 2: launch_canon_reg<R, 4>(j, ccols, pinv_eps, deps); return true;
+    case 3: launch_canon_reg<R, 6>(j, ccols, pinv_eps, deps); return true;
+    case 4: launch_canon_reg<R, 8>(j, ccols, pinv_eps, deps); return true;
+    case 5: launch_canon_reg<R, 10>(j, ccols, pinv_eps, deps); return true;
+    case 6: launch_canon_reg<R, 12>(j, ccols, pinv_eps, deps); return true;
+    case 7: launch_canon_reg<R, 14>(j, ccols, pinv_eps, deps); return true;
+    case 8: launch_canon_reg<R, 16>(j, ccols, pinv_eps, deps); return true;
+    default: return false;
+  }
+}
+
 template <typename R>
 void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const int chi = j->chi, n2 = 2 * chi;
@@ -485,8 +516,7 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const size_t wsb = canon_ws_bytes(n2);
   const double deps = j->single() ? (double)std::numeric_limits<float>::epsilon()
                                   : std::numeric_limits<double>::epsilon();
-  if (!fast_canon_launch<R>(reinterpret_cast<const cplx<R>*>(j->ext.p), j->E, chi, ccols, pinv_eps, deps,
-                            reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p), j->stream)) {
+  if (j->generic_only || !canon_reg_dispatch<R>(j, chi, ccols, pinv_eps, deps)) {
     int grid;
     size_t lds;
     unsigned char* ws = nullptr;
@@ -720,6 +750,10 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
     j->chimax = max_bond_dim;
     HIPX(hipGetDevice(&j->device));
     HIPX(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    {
+      const char* g = std::getenv("BQA_GENERIC_KERNELS");
+      j->generic_only = g && g[0] == '1';
+    }
     long total_nodes = 0, total_bonds = 0;
     for (int c = 0; c < classes; ++c) {
       require(class_degree[c] >= 0 && class_size[c] >= 0, "negative class degree or size");

@@ -19,11 +19,6 @@ bool fast_ext_launch(ClassView, int, const cplx<R>*, const cplx<R>*, cplx<R>*, d
 }
 
 template <typename R>
-bool fast_canon_launch(const cplx<R>*, int, int, int, double, double, cplx<double>*, double*, hipStream_t) {
-  return false;
-}
-
-template <typename R>
 bool fast_local_launch(ClassView, int, int, const cplx<R>*, cplx<R>*, const cplx<double>*, int, int, double, int,
                        double, double, const double*, int, cplx<R>*, hipStream_t) {
   return false;
