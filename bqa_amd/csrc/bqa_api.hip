@@ -391,7 +391,7 @@ void launch_bp_class(bqa_job* j, ClassDev& c) {
     e1 = take_event(j);
     HIPX(hipEventRecord(e0, j->stream));
   }
-  if (!fast_bp_launch<R>(c.view(), chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
+  if (j->generic_only || !fast_bp_launch<R>(c.view(), chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
                          msg_ptr<R>(j, 1), reinterpret_cast<BpStatus*>(j->status.p), j->stream)) {
     const long S = 2 * ipow_l(chi, c.d);
     const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
@@ -498,7 +498,7 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
     if (c.d == 0 || c.n == 0) continue;
     const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
     cplx<R>* ext = reinterpret_cast<cplx<R>*>(j->ext.p);
-    if (fast_ext_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), ext, ztime, j->stream)) {
+    if (!j->generic_only && fast_ext_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), ext, ztime, j->stream)) {
       LAUNCH_CHECK();
       continue;
     }
@@ -568,7 +568,7 @@ void local_update(bqa_job* j, int flags, int chi_out, double zz_time, double z_t
     ensure_tensor(j, c, dst, chi_out);
     const cplx<R>* Tin = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
     cplx<R>* Tout = reinterpret_cast<cplx<R>*>(c.T[dst].p);
-    if (!fast_local_launch<R>(c.view(), chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
+    if (j->generic_only || !fast_local_launch<R>(c.view(), chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
                               n2, ccols, zz_time, flags, z_time, x_time,
                               reinterpret_cast<const double*>(j->lmbd.p), j->lstride, msg_ptr<R>(j, j->mcur),
                               j->stream)) {
@@ -605,7 +605,7 @@ void density_impl(bqa_job* j, double* rho_out, double* bloch_out) {
   for (auto& c : j->cls) {
     if (c.n == 0) continue;
     const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
-    if (fast_density_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), reinterpret_cast<double*>(j->rho.p),
+    if (!j->generic_only && fast_density_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), reinterpret_cast<double*>(j->rho.p),
                                reinterpret_cast<double*>(j->bloch.p), j->stream)) {
       LAUNCH_CHECK();
       continue;

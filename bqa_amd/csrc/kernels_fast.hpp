@@ -5,6 +5,7 @@
 
 #include "common.hpp"
 #include "kernels_node.hpp"
+#include "kernels_mfma.hpp"
 
 namespace bqa {
 
@@ -27,6 +28,50 @@ bool fast_local_launch(ClassView, int, int, const cplx<R>*, cplx<R>*, const cplx
 template <typename R>
 bool fast_density_launch(ClassView, int, const cplx<R>*, const cplx<R>*, double*, double*, hipStream_t) {
   return false;
+}
+
+// FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp).
+inline int mfma8_grid(int n) { return n < 4096 ? n : 4096; }
+
+template <>
+inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB,
+                                  BpStatus* st, hipStream_t s) {
+  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
+  if (cv.d == 3)
+    hipLaunchKernelGGL((k_mfma8<3, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
+                       (cplx<float>*)nullptr, 0.0, (double*)nullptr, (double*)nullptr);
+  else
+    hipLaunchKernelGGL((k_mfma8<2, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
+                       (cplx<float>*)nullptr, 0.0, (double*)nullptr, (double*)nullptr);
+  return true;
+}
+
+template <>
+inline bool fast_ext_launch<float>(ClassView cv, int chi, const cplx<float>* T, const cplx<float>* msgs,
+                                   cplx<float>* ext, double ztime, hipStream_t s) {
+  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
+  cplx<float>* m = const_cast<cplx<float>*>(msgs);
+  if (cv.d == 3)
+    hipLaunchKernelGGL((k_mfma8<3, kModeExt>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
+                       (BpStatus*)nullptr, ext, ztime, (double*)nullptr, (double*)nullptr);
+  else
+    hipLaunchKernelGGL((k_mfma8<2, kModeExt>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
+                       (BpStatus*)nullptr, ext, ztime, (double*)nullptr, (double*)nullptr);
+  return true;
+}
+
+template <>
+inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>* T, const cplx<float>* msgs,
+                                       double* rho, double* bloch, hipStream_t s) {
+  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
+  cplx<float>* m = const_cast<cplx<float>*>(msgs);
+  if (cv.d == 3)
+    hipLaunchKernelGGL((k_mfma8<3, kModeDensity>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
+                       (BpStatus*)nullptr, (cplx<float>*)nullptr, 0.0, rho, bloch);
+  else
+    hipLaunchKernelGGL((k_mfma8<2, kModeDensity>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
+                       (BpStatus*)nullptr, (cplx<float>*)nullptr, 0.0, rho, bloch);
+  return true;
 }
 
 }  // namespace bqa

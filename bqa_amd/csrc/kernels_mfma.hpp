@@ -1,0 +1,290 @@
+// FP32 MFMA node kernels for bond dimension chi = 8 and degree 2 or 3.
+//
+// One wave per node.  The node tensor T[p, b_1..b_d] (complex64) is staged in
+// LDS with two work buffers.  Complex contractions run as real GEMMs on
+// v_mfma_f32_16x16x4f32 with complex numbers embedded as 2x2 real blocks:
+//
+//   mode product along bond j:  rows = the 2 chi^(d-1) fibres along j,
+//     [Yr | Yi] = [Xr | Xi] [[m_r^T, m_i^T], [-m_i^T, m_r^T]]      (K = N = 16)
+//   Gram with bond k free:  G = [Tr | Ti]^T [Yr | Yi]  (16 x 16, K = fibres),
+//     out_r = G[c][e] + G[8+c][8+e],  out_i = G[c][8+e] - G[8+c][e].
+//
+// MFMA 16x16x4 lane layout (checked on gfx950): A[i][k] at lane i + 16k,
+// B[k][j] at lane j + 16k, D[4 (l / 16) + v][l % 16] in result register v.
+// A mode product reads a 16-fibre tile completely before writing it, so it
+// may run in place.
+#pragma once
+
+#include "common.hpp"
+#include "kernels_node.hpp"
+
+namespace bqa {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+constexpr int cpow8(int e) { return e <= 0 ? 1 : 8 * cpow8(e - 1); }
+
+template <int D>
+struct Mf8 {
+  static constexpr int A = cpow8(D);      // chi^d
+  static constexpr int S = 2 * A;         // complex elements per node
+  static constexpr int ROWS = S / 8;      // fibres along one bond
+  static constexpr int TILES = ROWS / 16;
+  static constexpr int KS = ROWS / 4;     // k-steps of a Gram
+};
+
+template <int D, int J>
+__device__ __forceinline__ int fbase8(int r) {
+  constexpr int st = cpow8(D - 1 - J);
+  return (r / st) * 8 * st + (r % st);
+}
+
+__device__ __forceinline__ f4v mfma4(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// B operand of the real embedding of message m for k-step s (lane-dependent).
+__device__ __forceinline__ void msg_operand8(const cplx<float>* __restrict__ m, float (&bm)[4]) {
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int a = i16 & 7, cn = i16 >> 3;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kr = 4 * s + kq;
+    const int b = kr & 7, ck = kr >> 3;
+    const cplx<float> v = m[a * 8 + b];
+    bm[s] = ck == 0 ? (cn == 0 ? v.x : v.y) : (cn == 0 ? -v.y : v.x);
+  }
+}
+
+// Y = X x_J m (float views of complex64 LDS arrays; X may equal Y).
+template <int D, int J>
+__device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[4]) {
+  constexpr int st = cpow8(D - 1 - J);
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int col = i16 & 7, comp = i16 >> 3;
+#pragma unroll
+  for (int t = 0; t < Mf8<D>::TILES; ++t) {
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    const int base = fbase8<D, J>(t * 16 + i16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kr = 4 * s + kq;
+      acc = mfma4(X[2 * (base + (kr & 7) * st) + (kr >> 3)], bm[s], acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bb = fbase8<D, J>(t * 16 + 4 * kq + v);
+      Y[2 * (bb + col * st) + comp] = acc[v];
+    }
+  }
+}
+
+// G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J; with PSPLIT, g0
+// sums the p = 0 fibres and g1 the p = 1 fibres.
+template <int D, int J, bool PSPLIT>
+__device__ __forceinline__ void gram8(const float* T, const float* Y, f4v& g0, f4v& g1) {
+  constexpr int st = cpow8(D - 1 - J);
+  constexpr int KS = Mf8<D>::KS;
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int off = (i16 & 7) * st, comp = i16 >> 3;
+  g0 = f4v{0.f, 0.f, 0.f, 0.f};
+  g1 = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+  for (int s = 0; s < KS; ++s) {
+    const int idx = 2 * (fbase8<D, J>(4 * s + kq) + off) + comp;
+    const float a = T[idx], b = Y[idx];
+    if (!PSPLIT || s < KS / 2)
+      g0 = mfma4(a, b, g0);
+    else
+      g1 = mfma4(a, b, g1);
+  }
+}
+
+__device__ __forceinline__ void store_g(float* Gs, const f4v& g) {
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Gs[(4 * kq + v) * 16 + i16] = g[v];
+}
+
+// out[c][e] from a 16x16 real Gram stored row-major in Gs.
+__device__ __forceinline__ cplx<float> gram_entry(const float* Gs, int c, int e) {
+  return mk(Gs[c * 16 + e] + Gs[(8 + c) * 16 + 8 + e], Gs[c * 16 + 8 + e] - Gs[(8 + c) * 16 + e]);
+}
+
+__device__ __forceinline__ cplx<float> gram_trace(const float* Gs) {
+  cplx<float> t = mk(0.f, 0.f);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) t = t + gram_entry(Gs, c, c);
+  return t;
+}
+
+template <int D>
+struct Mf8Smem {
+  float T[2 * Mf8<D>::S];
+  float W1[2 * Mf8<D>::S];
+  float W2[2 * Mf8<D>::S];
+  float G[2][256];
+};
+
+// Finish one outgoing message from its Gram: BP mode (trace-normalised chi x chi,
+// dist partials) or extended mode (2chi x 2chi with the ZZ split).
+template <int MODE>
+__device__ __forceinline__ void finish8(float* G0s, float* G1s, const f4v& g0,
+                                        const f4v& g1, long pos, const cplx<float>* min, cplx<float>* mout,
+                                        cplx<float>* ext_out, double theta, double& dnum, double& dden) {
+  const int l = threadIdx.x & 63;
+  store_g(G0s, g0);
+  if (MODE == kModeExt) store_g(G1s, g1);
+  __syncthreads();
+  if (MODE == kModeBP) {
+    const cplx<float> inv = cinv(gram_trace(G0s));
+    const cplx<float> v = gram_entry(G0s, l >> 3, l & 7) * inv;
+    const cplx<float> old = min[pos * 64 + l];
+    dnum = fmax(dnum, (double)cabs(v - old));
+    dden = fmax(dden, (double)cabs(v + old));
+    mout[pos * 64 + l] = v;
+  } else {
+    cplx<double> gd0, gd1;
+    zz_split(theta, gd0, gd1);
+    const cplx<double> trd = cast<double>(gram_trace(G0s) + gram_trace(G1s));
+    const cplx<double> inv = cinv(trd * mk(abs2(gd0) + abs2(gd1), 0.0));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int o = l + 64 * q;
+      const int row = o >> 4, col = o & 15;
+      const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
+      const cplx<float> a0 = gram_entry(G0s, c, e), a1 = gram_entry(G1s, c, e);
+      const cplx<float> gsum = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
+      const cplx<double> ga = s ? gd1 : gd0, gb = s2 ? gd1 : gd0;
+      ext_out[pos * 256 + o] = cast<float>(conj(ga) * gb * cast<double>(gsum) * inv);
+    }
+  }
+  __syncthreads();
+}
+
+template <int D, int MODE>
+__global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
+                                              cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
+                                              cplx<float>* ext_out, double ztime, double* rho_out,
+                                              double* bloch_out) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  __shared__ __align__(16) Mf8Smem<D> sm;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63;
+  cplx<float>* min = msgA;
+  cplx<float>* mout = msgB;
+  if (MODE == kModeBP) {
+    if (st->done) return;
+    const int cur = st->cur;
+    min = cur ? msgB : msgA;
+    mout = cur ? msgA : msgB;
+  }
+  double dnum = 0.0, dden = 0.0;
+  constexpr int S = Mf8<D>::S;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const float4* Tg = reinterpret_cast<const float4*>(T + (size_t)i * S);
+    float4* Ts = reinterpret_cast<float4*>(sm.T);
+#pragma unroll
+    for (int x = l; x < S / 2; x += 64) Ts[x] = Tg[x];
+    float bm[D][4];
+#pragma unroll
+    for (int j = 0; j < D; ++j) msg_operand8(min + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
+    __syncthreads();
+    if (MODE == kModeDensity) {
+      mp8<D, 0>(sm.T, sm.W1, bm[0]);
+      __syncthreads();
+      mp8<D, 1>(sm.W1, sm.W1, bm[1]);
+      __syncthreads();
+      if constexpr (D == 3) {
+        mp8<D, 2>(sm.W1, sm.W1, bm[2]);
+        __syncthreads();
+      }
+      // rho[p][q] = sum_a TM[p, a] conj(T[q, a])
+      constexpr int A = Mf8<D>::A;
+      cplx<float> r[4] = {mk(0.f, 0.f), mk(0.f, 0.f), mk(0.f, 0.f), mk(0.f, 0.f)};
+      const cplx<float>* TM = reinterpret_cast<const cplx<float>*>(sm.W1);
+      const cplx<float>* Tc = reinterpret_cast<const cplx<float>*>(sm.T);
+      for (int a = l; a < A; a += 64) {
+        cfma_cj(r[0], Tc[a], TM[a]);
+        cfma_cj(r[1], Tc[A + a], TM[a]);
+        cfma_cj(r[2], Tc[a], TM[A + a]);
+        cfma_cj(r[3], Tc[A + a], TM[A + a]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        for (int o = 32; o > 0; o >>= 1) {
+          r[q].x += __shfl_xor(r[q].x, o, 64);
+          r[q].y += __shfl_xor(r[q].y, o, 64);
+        }
+      if (l == 0) {
+        cplx<double> r00 = cast<double>(r[0]), r01 = cast<double>(r[1]);
+        cplx<double> r10 = cast<double>(r[2]), r11 = cast<double>(r[3]);
+        const cplx<double> inv = cinv(r00 + r11);
+        r00 = r00 * inv;
+        r01 = r01 * inv;
+        r10 = r10 * inv;
+        r11 = r11 * inv;
+        const long id = cv.node_ids[i];
+        if (rho_out) {
+          double* ro = rho_out + id * 8;
+          ro[0] = r00.x; ro[1] = r00.y; ro[2] = r01.x; ro[3] = r01.y;
+          ro[4] = r10.x; ro[5] = r10.y; ro[6] = r11.x; ro[7] = r11.y;
+        }
+        if (bloch_out) {
+          double* bo = bloch_out + id * 3;
+          bo[0] = r01.x + r10.x;
+          bo[1] = r10.y - r01.y;
+          bo[2] = r00.x - r11.x;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    constexpr bool PS = MODE == kModeExt;
+    f4v g0, g1;
+    if constexpr (D == 2) {
+      mp8<D, 1>(sm.T, sm.W1, bm[1]);          // Y0 = T x1 m1
+      __syncthreads();
+      gram8<D, 0, PS>(sm.T, sm.W1, g0, g1);
+      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], min, mout, ext_out,
+                    cv.J[0 * n + i] * ztime, dnum, dden);
+      mp8<D, 0>(sm.T, sm.W2, bm[0]);          // Y1 = T x0 m0
+      __syncthreads();
+      gram8<D, 1, PS>(sm.T, sm.W2, g0, g1);
+      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], min, mout, ext_out,
+                    cv.J[1 * n + i] * ztime, dnum, dden);
+    } else {
+      mp8<D, 2>(sm.T, sm.W1, bm[2]);          // W1 = T x2 m2
+      __syncthreads();
+      mp8<D, 1>(sm.W1, sm.W2, bm[1]);         // Y0 = W1 x1 m1
+      __syncthreads();
+      gram8<D, 0, PS>(sm.T, sm.W2, g0, g1);
+      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], min, mout, ext_out,
+                    cv.J[0 * n + i] * ztime, dnum, dden);
+      mp8<D, 0>(sm.W1, sm.W2, bm[0]);         // Y1 = W1 x0 m0
+      __syncthreads();
+      gram8<D, 1, PS>(sm.T, sm.W2, g0, g1);
+      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], min, mout, ext_out,
+                    cv.J[1 * n + i] * ztime, dnum, dden);
+      mp8<D, 0>(sm.T, sm.W1, bm[0]);          // W1 = T x0 m0
+      __syncthreads();
+      mp8<D, 1>(sm.W1, sm.W1, bm[1]);         // Y2 = W1 x1 m1 (in place)
+      __syncthreads();
+      gram8<D, 2, PS>(sm.T, sm.W1, g0, g1);
+      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[2 * n + i], min, mout, ext_out,
+                    cv.J[2 * n + i] * ztime, dnum, dden);
+    }
+  }
+  if (MODE == kModeBP) {
+    for (int o = 32; o > 0; o >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+      dden = fmax(dden, __shfl_xor(dden, o, 64));
+    }
+    if (l == 0) {
+      atomic_max_nonneg(&st->num_bits, dnum);
+      atomic_max_nonneg(&st->den_bits, dden);
+    }
+  }
+}
+
+}  // namespace bqa

@@ -214,3 +214,90 @@ def test_energy_of_rounded_spins():
     assert np.array_equal(spins, s)
     want = sum(h * s[v] for v, h in nodes.items()) + sum(j * s[a] * s[b] for (a, b), j in edges.items())
     assert abs(e - want) < 1e-9 * max(1.0, abs(want))
+
+
+def _run_bloch(cfg, precision, generic):
+    import os
+    old = os.environ.get("BQA_GENERIC_KERNELS")
+    os.environ["BQA_GENERIC_KERNELS"] = "1" if generic else "0"
+    try:
+        bloch, st, _ = run_product(cfg, precision)
+        return bloch, st
+    finally:
+        if old is None:
+            os.environ.pop("BQA_GENERIC_KERNELS", None)
+        else:
+            os.environ["BQA_GENERIC_KERNELS"] = old
+
+
+FAST_CASES = {
+    "rr60_chi8": lambda: instances.regular_uniform(60, 3, seed=21, field_sigma=0.4),
+    "grid5x7_chi8": lambda: instances.grid_gaussian(5, 7, seed=22, field_sigma=0.1),
+    "heavyhex_chi8": lambda: instances.heavy_hex_gaussian(3, 4, seed=23, field_sigma=0.5),
+    "rr40_chi4": lambda: instances.regular_uniform(40, 3, seed=24, field_sigma=0.4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FAST_CASES))
+def test_fast_kernels_end_to_end(name):
+    """Specialised kernels end to end: FP64 equals the oracle; FP32 stays within 1e-3 of it
+    (FP32 rounding of the extended messages is amplified by S^-1/2 on Schmidt values near
+    pinv_eps, so sensitive instances drift further than the golden fixtures)."""
+    nodes, edges = FAST_CASES[name]()
+    chi = 4 if name.endswith("chi4") else 8
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "bp_eps": 1e-9, "max_bp_iter_number": 60,
+           "schedule": instances.anneal_schedule(2.0, 24)}
+    want = np.array(O.run_qa(dict(cfg))[0][1])
+    print(f"[gpu] fast end-to-end {name}", flush=True)
+    fast32, st = _run_bloch(cfg, "single", generic=False)
+    fast64, _ = _run_bloch(cfg, "double", generic=False)
+    print(f"[gpu]   chis {st.chis[-3:]} |fp32-oracle| {np.abs(fast32 - want).max():.2e} "
+          f"|fp64-oracle| {np.abs(fast64 - want).max():.2e}", flush=True)
+    assert np.abs(fast32 - want).max() < 1e-3
+    assert np.abs(fast64 - want).max() < 1e-8
+
+
+def _job_from_oracle(cfg, ctx_o, st_o, precision, generic):
+    import os
+    os.environ["BQA_GENERIC_KERNELS"] = "1" if generic else "0"
+    try:
+        job = Job(config_to_context(dict(cfg, backend="mi355x")), precision)
+        job.init_state()
+        _upload(job, ctx_o, st_o)
+        return job
+    finally:
+        os.environ.pop("BQA_GENERIC_KERNELS", None)
+
+
+@pytest.mark.parametrize("name", sorted(FAST_CASES))
+def test_fast_kernel_phases_match_generic_fp32(name):
+    """One state uploaded into a fast-kernel job and a generic-kernel job (both FP32):
+    a single BP sweep, the density readout and the simple update's lambdas agree."""
+    nodes, edges = FAST_CASES[name]()
+    chi = 4 if name.endswith("chi4") else 8
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "bp_eps": 1e-9, "max_bp_iter_number": 60,
+           "schedule": instances.anneal_schedule(2.0, 24)}
+    ctx_o = O.make_context(cfg)
+    st_o = O.init_state(ctx_o)
+    layers = [i for i in ctx_o.instructions if isinstance(i, dict)]
+    k = 0
+    while st_o.chi < chi and k < len(layers) - 1:
+        O.run_layer(ctx_o, st_o, layers[k]["xtime"], layers[k]["ztime"])
+        k += 1
+    print(f"[gpu] fast-vs-generic phases {name} at step {k}, chi {st_o.chi}", flush=True)
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, "single", g) for g in (False, True)]
+    try:
+        m = []
+        for job in jobs:
+            job.run_bp(1, 0.0, 0.0)
+            m.append(job.get_messages())
+        assert np.abs(m[0] - m[1]).max() < 2e-6 * np.abs(m[1]).max()
+        r = [job.density()[0] for job in jobs]
+        assert np.abs(r[0] - r[1]).max() < 2e-6
+        su = [job.simple_update(layers[k]["ztime"], ctx_o.pinv_eps) for job in jobs]
+        assert su[0][0] == su[1][0]
+        lm = [np.sort(job.get_lambdas().real, axis=1) for job in jobs]
+        assert np.abs(lm[0] - lm[1]).max() < 1e-5
+    finally:
+        for job in jobs:
+            job.close()
