@@ -136,6 +136,8 @@ def accuracy(fast, ref):
     spin_mismatch = int(np.sum((fast["spins"] != ref["spins"]) & decided))
     e_rel = abs(fast["energy"] - ref["energy"]) / max(1.0, abs(ref["energy"]))
     return {"max_abs_dz": float(dz.max()), "mean_abs_dz": float(dz.mean()),
+            "p99_abs_dz": float(np.percentile(dz, 99)), "p999_abs_dz": float(np.percentile(dz, 99.9)),
+            "qubits_dz_above_1e-4": int(np.sum(dz > 1e-4)), "qubits": int(dz.size),
             "max_abs_dbloch": float(np.abs(fast["bloch"] - ref["bloch"]).max()),
             "spin_mismatches_decided": spin_mismatch, "energy_fp32": fast["energy"],
             "energy_fp64": ref["energy"], "energy_rel_err": e_rel}

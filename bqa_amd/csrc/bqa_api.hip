@@ -136,7 +136,7 @@ struct bqa_job {
   int lstride = 1;
   int pred_sweeps = 8;
   std::vector<ClassDev> cls;
-  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws;
+  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws, conv, conv_msgs;
   BpStatus* st_host = nullptr;
   double* lmax_host = nullptr;
   hipStream_t stream = nullptr;
@@ -490,25 +490,68 @@ bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double 
   }
 }
 
+__global__ void k_widen(const cplx<float>* __restrict__ src, cplx<double>* dst, long n) {
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
+    dst[x] = cast<double>(src[x]);
+}
+
+// Extended messages of one class in FP64 arithmetic: MFMA for chi = 8 and degree 2-3,
+// otherwise the generic kernel (FP32 inputs are widened into a scratch buffer first).
+template <typename R>
+void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64) {
+  const int chi = j->chi;
+  const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
+  cplx<double>* ext = reinterpret_cast<cplx<double>*>(j->ext.p);
+  if (!j->generic_only && chi == 8 && (c.d == 2 || c.d == 3)) {
+    const int grid = std::min(c.n, 4096);
+    if (c.d == 3)
+      hipLaunchKernelGGL((k_mfma8_ext64<3, R>), dim3(grid), dim3(64), 0, j->stream, c.view(), T,
+                         msg_ptr<R>(j, j->mcur), ext, ztime);
+    else
+      hipLaunchKernelGGL((k_mfma8_ext64<2, R>), dim3(grid), dim3(64), 0, j->stream, c.view(), T,
+                         msg_ptr<R>(j, j->mcur), ext, ztime);
+    LAUNCH_CHECK();
+    return;
+  }
+  const long S = 2 * ipow_l(chi, c.d);
+  const cplx<double>* T64 = reinterpret_cast<const cplx<double>*>(T);
+  if (sizeof(R) == sizeof(float)) {
+    const size_t tb = (size_t)c.n * S * sizeof(cplx<double>);
+    j->conv.ensure(tb, j->dev_bytes, j->stream);
+    hipLaunchKernelGGL(k_widen, dim3(grid_for((long)c.n * S)), dim3(256), 0, j->stream,
+                       reinterpret_cast<const cplx<float>*>(T), reinterpret_cast<cplx<double>*>(j->conv.p),
+                       (long)c.n * S);
+    LAUNCH_CHECK();
+    T64 = reinterpret_cast<const cplx<double>*>(j->conv.p);
+  }
+  const size_t fixed = sizeof(cplx<double>) * (size_t)(kNT + 4 * chi * chi);
+  NodeLaunch L = plan_node(j, c.n, sizeof(cplx<double>) * 3 * S, fixed, 3 * S, sizeof(cplx<double>));
+  hipLaunchKernelGGL((k_node_contract<double, kModeExt>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi,
+                     T64, const_cast<cplx<double>*>(msgs64), (cplx<double>*)nullptr, (BpStatus*)nullptr, ext, ztime,
+                     (double*)nullptr, (double*)nullptr, reinterpret_cast<cplx<double>*>(L.ws), L.ws_elems);
+  LAUNCH_CHECK();
+}
+
 template <typename R>
 void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const int chi = j->chi, n2 = 2 * chi;
-  j->ext.ensure((size_t)j->M * n2 * n2 * j->csize, j->dev_bytes, j->stream);
+  j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
+  // FP64 view of the current messages for the generic extended-message kernel
+  const cplx<double>* msgs64 = reinterpret_cast<const cplx<double>*>(msg_ptr<R>(j, j->mcur));
+  bool need_generic = j->generic_only || chi != 8;
+  for (auto& c : j->cls) need_generic |= (c.d != 2 && c.d != 3 && c.d != 0);
+  if (sizeof(R) == sizeof(float) && need_generic) {
+    const long cnt = (long)j->M * chi * chi;
+    j->conv_msgs.ensure((size_t)cnt * sizeof(cplx<double>), j->dev_bytes, j->stream);
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(cnt)), dim3(256), 0, j->stream,
+                       reinterpret_cast<const cplx<float>*>(msg_ptr<R>(j, j->mcur)),
+                       reinterpret_cast<cplx<double>*>(j->conv_msgs.p), cnt);
+    LAUNCH_CHECK();
+    msgs64 = reinterpret_cast<const cplx<double>*>(j->conv_msgs.p);
+  }
   for (auto& c : j->cls) {
     if (c.d == 0 || c.n == 0) continue;
-    const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
-    cplx<R>* ext = reinterpret_cast<cplx<R>*>(j->ext.p);
-    if (!j->generic_only && fast_ext_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), ext, ztime, j->stream)) {
-      LAUNCH_CHECK();
-      continue;
-    }
-    const long S = 2 * ipow_l(chi, c.d);
-    const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
-    NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 3 * S, fixed, 3 * S, sizeof(cplx<R>));
-    hipLaunchKernelGGL((k_node_contract<R, kModeExt>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi, T,
-                       msg_ptr<R>(j, j->mcur), (cplx<R>*)nullptr, (BpStatus*)nullptr, ext, ztime, (double*)nullptr,
-                       (double*)nullptr, reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
-    LAUNCH_CHECK();
+    ext_class<R>(j, c, ztime, msgs64);
   }
   const int ccols = std::min(n2, j->chimax);
   j->canon.ensure((size_t)j->M * n2 * ccols * sizeof(zd), j->dev_bytes, j->stream);
@@ -516,7 +559,7 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const size_t wsb = canon_ws_bytes(n2);
   const double deps = j->single() ? (double)std::numeric_limits<float>::epsilon()
                                   : std::numeric_limits<double>::epsilon();
-  if (j->generic_only || !canon_reg_dispatch<R>(j, chi, ccols, pinv_eps, deps)) {
+  if (j->generic_only || !canon_reg_dispatch<double>(j, chi, ccols, pinv_eps, deps)) {
     int grid;
     size_t lds;
     unsigned char* ws = nullptr;
@@ -529,8 +572,8 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
       ws = reinterpret_cast<unsigned char*>(j->cws.p);
       lds = 0;
     }
-    hipLaunchKernelGGL((k_canonicalize<R>), dim3(grid), dim3(kCanonNT), lds, j->stream,
-                       reinterpret_cast<const cplx<R>*>(j->ext.p), j->E, chi, ccols, pinv_eps, deps,
+    hipLaunchKernelGGL((k_canonicalize<double>), dim3(grid), dim3(kCanonNT), lds, j->stream,
+                       reinterpret_cast<const cplx<double>*>(j->ext.p), j->E, chi, ccols, pinv_eps, deps,
                        reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p), ws, (long)wsb);
     LAUNCH_CHECK();
   }
@@ -686,7 +729,7 @@ size_t estimate(int M, const std::vector<int>& deg, const std::vector<int>& size
   double b = 0.0;
   for (size_t c = 0; c < deg.size(); ++c) b += 2.0 * size[c] * 2.0 * ipow_d(chimax, deg[c]) * cs;
   b += 2.0 * M * (double)chimax * chimax * cs;                  // messages
-  b += (double)M * 4.0 * chimax * chimax * cs;                  // extended messages
+  b += (double)M * 4.0 * chimax * chimax * 16.0;                // extended messages (FP64)
   b += (double)M * 2.0 * chimax * chimax * sizeof(zd);          // canonicalisers
   b += (double)(M / 2) * 2.0 * chimax * sizeof(double);         // lambdas
   return (size_t)b;
@@ -843,7 +886,7 @@ int bqa_job_destroy(bqa_job* j) {
       c.T[1].release(j->dev_bytes);
     }
     DevBuf* bufs[] = {&j->msgs[0], &j->msgs[1], &j->lmbd, &j->ext, &j->canon, &j->lmax, &j->status,
-                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws};
+                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws, &j->conv, &j->conv_msgs};
     for (DevBuf* b : bufs) b->release(j->dev_bytes);
     for (auto& pr : j->ev_pending) {
       (void)hipEventDestroy(pr.first);

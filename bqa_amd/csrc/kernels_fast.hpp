@@ -15,11 +15,6 @@ bool fast_bp_launch(ClassView, int, const cplx<R>*, cplx<R>*, cplx<R>*, BpStatus
 }
 
 template <typename R>
-bool fast_ext_launch(ClassView, int, const cplx<R>*, const cplx<R>*, cplx<R>*, double, hipStream_t) {
-  return false;
-}
-
-template <typename R>
 bool fast_local_launch(ClassView, int, int, const cplx<R>*, cplx<R>*, const cplx<double>*, int, int, double, int,
                        double, double, const double*, int, cplx<R>*, hipStream_t) {
   return false;
@@ -43,20 +38,6 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
   else
     hipLaunchKernelGGL((k_mfma8<2, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
                        (cplx<float>*)nullptr, 0.0, (double*)nullptr, (double*)nullptr);
-  return true;
-}
-
-template <>
-inline bool fast_ext_launch<float>(ClassView cv, int chi, const cplx<float>* T, const cplx<float>* msgs,
-                                   cplx<float>* ext, double ztime, hipStream_t s) {
-  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
-  cplx<float>* m = const_cast<cplx<float>*>(msgs);
-  if (cv.d == 3)
-    hipLaunchKernelGGL((k_mfma8<3, kModeExt>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
-                       (BpStatus*)nullptr, ext, ztime, (double*)nullptr, (double*)nullptr);
-  else
-    hipLaunchKernelGGL((k_mfma8<2, kModeExt>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
-                       (BpStatus*)nullptr, ext, ztime, (double*)nullptr, (double*)nullptr);
   return true;
 }
 

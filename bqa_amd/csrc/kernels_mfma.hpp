@@ -288,3 +288,168 @@ __global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* _
 }
 
 }  // namespace bqa
+
+namespace bqa {
+
+// ---------------------------------------------------------------------------
+// Extended messages in FP64 arithmetic (FP32 or FP64 inputs), chi = 8.
+//
+// FP32 rounding inside the extended-message contraction is amplified by the
+// S^{-1/2} of the canonicalisers on small Schmidt values; computed in FP64
+// from the stored tensors and messages, the FP32 path's error drops to the
+// level set by the storage precision.  v_mfma_f64_16x16x4f64 has the same A
+// and B lane layout as the FP32 instruction; its result register v of lane l
+// holds D[l / 16 + 4 v][l % 16].
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma4d(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename R>
+__device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, double (&bm)[4]) {
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int a = i16 & 7, cn = i16 >> 3;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kr = 4 * s + kq;
+    const int b = kr & 7, ck = kr >> 3;
+    const cplx<R> v = m[a * 8 + b];
+    const double vr = (double)v.x, vi = (double)v.y;
+    bm[s] = ck == 0 ? (cn == 0 ? vr : vi) : (cn == 0 ? -vi : vr);
+  }
+}
+
+template <int D, int J>
+__device__ __forceinline__ void mp8d(const double* X, double* Y, const double (&bm)[4]) {
+  constexpr int st = cpow8(D - 1 - J);
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int col = i16 & 7, comp = i16 >> 3;
+#pragma unroll
+  for (int t = 0; t < Mf8<D>::TILES; ++t) {
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    const int base = fbase8<D, J>(t * 16 + i16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kr = 4 * s + kq;
+      acc = mfma4d(X[2 * (base + (kr & 7) * st) + (kr >> 3)], bm[s], acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bb = fbase8<D, J>(t * 16 + kq + 4 * v);
+      Y[2 * (bb + col * st) + comp] = acc[v];
+    }
+  }
+}
+
+template <int D, int J>
+__device__ __forceinline__ void gram8d(const double* T, const double* Y, d4v& g0, d4v& g1) {
+  constexpr int st = cpow8(D - 1 - J);
+  constexpr int KS = Mf8<D>::KS;
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int off = (i16 & 7) * st, comp = i16 >> 3;
+  g0 = d4v{0.0, 0.0, 0.0, 0.0};
+  g1 = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+  for (int s = 0; s < KS; ++s) {
+    const int idx = 2 * (fbase8<D, J>(4 * s + kq) + off) + comp;
+    if (s < KS / 2)
+      g0 = mfma4d(T[idx], Y[idx], g0);
+    else
+      g1 = mfma4d(T[idx], Y[idx], g1);
+  }
+}
+
+__device__ __forceinline__ void store_gd(double* Gs, const d4v& g) {
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Gs[(kq + 4 * v) * 16 + i16] = g[v];
+}
+
+__device__ __forceinline__ cplx<double> gram_entry_d(const double* Gs, int c, int e) {
+  return mk(Gs[c * 16 + e] + Gs[(8 + c) * 16 + 8 + e], Gs[c * 16 + 8 + e] - Gs[(8 + c) * 16 + e]);
+}
+
+template <int D>
+struct Mf8SmemD {
+  double T[2 * Mf8<D>::S];
+  double W1[2 * Mf8<D>::S];
+  double W2[2 * Mf8<D>::S];
+  double G[2][256];
+};
+
+// Write one 2chi x 2chi extended message (FP64) from the per-p Grams.
+__device__ __forceinline__ void finish8_ext64(double* G0s, double* G1s, const d4v& g0, const d4v& g1, long pos,
+                                              cplx<double>* ext_out, double theta) {
+  const int l = threadIdx.x & 63;
+  store_gd(G0s, g0);
+  store_gd(G1s, g1);
+  __syncthreads();
+  cplx<double> g0c, g1c;
+  zz_split(theta, g0c, g1c);
+  cplx<double> tr = mk(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) tr = tr + gram_entry_d(G0s, c, c) + gram_entry_d(G1s, c, c);
+  const cplx<double> inv = cinv(tr * mk(abs2(g0c) + abs2(g1c), 0.0));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int o = l + 64 * q;
+    const int row = o >> 4, col = o & 15;
+    const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
+    const cplx<double> a0 = gram_entry_d(G0s, c, e), a1 = gram_entry_d(G1s, c, e);
+    const cplx<double> gsum = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
+    const cplx<double> ga = s ? g1c : g0c, gb = s2 ? g1c : g0c;
+    ext_out[pos * 256 + o] = conj(ga) * gb * gsum * inv;
+  }
+  __syncthreads();
+}
+
+template <int D, typename R>
+__global__ __launch_bounds__(64) void k_mfma8_ext64(ClassView cv, const cplx<R>* __restrict__ T,
+                                                    const cplx<R>* __restrict__ msgs, cplx<double>* ext_out,
+                                                    double ztime) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  __shared__ __align__(16) Mf8SmemD<D> sm;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63;
+  constexpr int S = Mf8<D>::S;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<R>* Tg = T + (size_t)i * S;
+    cplx<double>* Ts = reinterpret_cast<cplx<double>*>(sm.T);
+    for (int x = l; x < S; x += 64) Ts[x] = cast<double>(Tg[x]);
+    double bm[D][4];
+#pragma unroll
+    for (int j = 0; j < D; ++j) msg_operand8d(msgs + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
+    __syncthreads();
+    d4v g0, g1;
+    if constexpr (D == 2) {
+      mp8d<D, 1>(sm.T, sm.W1, bm[1]);
+      __syncthreads();
+      gram8d<D, 0>(sm.T, sm.W1, g0, g1);
+      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime);
+      mp8d<D, 0>(sm.T, sm.W2, bm[0]);
+      __syncthreads();
+      gram8d<D, 1>(sm.T, sm.W2, g0, g1);
+      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime);
+    } else {
+      mp8d<D, 2>(sm.T, sm.W1, bm[2]);
+      __syncthreads();
+      mp8d<D, 1>(sm.W1, sm.W2, bm[1]);
+      __syncthreads();
+      gram8d<D, 0>(sm.T, sm.W2, g0, g1);
+      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime);
+      mp8d<D, 0>(sm.W1, sm.W2, bm[0]);
+      __syncthreads();
+      gram8d<D, 1>(sm.T, sm.W2, g0, g1);
+      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime);
+      mp8d<D, 0>(sm.T, sm.W1, bm[0]);
+      __syncthreads();
+      mp8d<D, 1>(sm.W1, sm.W1, bm[1]);
+      __syncthreads();
+      gram8d<D, 2>(sm.T, sm.W1, g0, g1);
+      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[2 * n + i], ext_out, cv.J[2 * n + i] * ztime);
+    }
+  }
+}
+
+}  // namespace bqa
