@@ -15,8 +15,8 @@ the timed region starts.
 The FP32 path is timed.  Accuracy: the same anneal in FP64 on the GPU
 (untimed) gives the per-qubit <Z> error, rounded-spin disagreements and the
 energy error.  cpu_baseline: the FP64 NumPy oracle on a bounded sample of the
-same workload (1,000-qubit instance from the same generator, three Trotter
-steps after the chi ramp, one core).
+same workload (3,000-qubit instance from the same generator, six Trotter
+steps once the bond dimension reaches chi, one core).
 """
 from __future__ import annotations
 
@@ -95,13 +95,13 @@ def load_traffic():
 
 
 def cpu_baseline(chi):
-    """FP64 NumPy oracle, one thread, 1,000-qubit sample of the same workload."""
+    """FP64 NumPy oracle, one thread, 3,000-qubit sample of the same workload."""
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:              # pragma: no cover
         threadpool_limits = None
     from oracle import bqa_oracle as O
-    n = 1000
+    n = 3000
     cfg = make_config(n, chi, BASE_STEPS, "double")
     ctx = O.make_context(cfg)
     st = O.init_state(ctx)
@@ -113,7 +113,7 @@ def cpu_baseline(chi):
             ins = layers[ramp]
             O.run_layer(ctx, st, ins["xtime"], ins["ztime"])
             ramp += 1
-        s_timed = 3
+        s_timed = 6
         t0 = time.perf_counter()
         for ins in layers[ramp:ramp + s_timed]:
             O.run_layer(ctx, st, ins["xtime"], ins["ztime"])

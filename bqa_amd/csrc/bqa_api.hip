@@ -21,6 +21,7 @@
 #include "../../include/bqa_hip.h"
 #include "common.hpp"
 #include "kernels_canon.hpp"
+#include "kernels_canon_h.hpp"
 #include "kernels_canon_reg.hpp"
 #include "kernels_fast.hpp"
 #include "kernels_node.hpp"
@@ -149,7 +150,8 @@ struct bqa_job {
   int64_t t_launches = 0;
   double t_ms = 0.0, t_bytes = 0.0;
   std::vector<double> pending_bytes;
-  bool generic_only = false;  // BQA_GENERIC_KERNELS=1: run the generic kernels for every shape
+  bool generic_only = false;        // BQA_GENERIC_KERNELS=1: run the generic kernels for every shape
+  bool canon_full_columns = false;  // BQA_CANON_REG=1: full-column register canonicaliser
 
   bool single() const { return prec == BQA_PREC_SINGLE; }
 };
@@ -474,9 +476,35 @@ void launch_canon_reg(bqa_job* j, int ccols, double pinv_eps, double deps) {
   LAUNCH_CHECK();
 }
 
-// Register-resident canonicalisers for chi <= 16; false for larger chi.
+template <int N>
+void launch_canon_h(bqa_job* j, int ccols, double pinv_eps, double deps) {
+  using S = HCanon<N>;
+  j->cws.ensure((size_t)j->E * S::WS * sizeof(double), j->dev_bytes, j->stream);
+  const int grid = (j->E + S::EPW - 1) / S::EPW;
+  hipLaunchKernelGGL((k_canon_h<N>), dim3(grid), dim3(64), 0, j->stream,
+                     reinterpret_cast<const cplx<double>*>(j->ext.p), j->E, ccols, pinv_eps, deps,
+                     reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p),
+                     reinterpret_cast<double*>(j->cws.p));
+  LAUNCH_CHECK();
+}
+
+// Register-resident canonicalisers for chi <= 8 (split-column Cholesky + Jacobi kernel unless
+// BQA_CANON_REG=1 selects the full-column kernel); false for larger chi.
 template <typename R>
 bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double deps) {
+  if (!j->canon_full_columns) {
+    switch (chi) {
+      case 1: launch_canon_h<2>(j, ccols, pinv_eps, deps); return true;
+      case 2: launch_canon_h<4>(j, ccols, pinv_eps, deps); return true;
+      case 3: launch_canon_h<6>(j, ccols, pinv_eps, deps); return true;
+      case 4: launch_canon_h<8>(j, ccols, pinv_eps, deps); return true;
+      case 5: launch_canon_h<10>(j, ccols, pinv_eps, deps); return true;
+      case 6: launch_canon_h<12>(j, ccols, pinv_eps, deps); return true;
+      case 7: launch_canon_h<14>(j, ccols, pinv_eps, deps); return true;
+      case 8: launch_canon_h<16>(j, ccols, pinv_eps, deps); return true;
+      default: return false;
+    }
+  }
   switch (chi) {
     case 1: launch_canon_reg<R, 2>(j, ccols, pinv_eps, deps); return true;
     case 2: launch_canon_reg<R, 4>(j, ccols, pinv_eps, deps); return true;
@@ -796,6 +824,8 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
     {
       const char* g = std::getenv("BQA_GENERIC_KERNELS");
       j->generic_only = g && g[0] == '1';
+      const char* c = std::getenv("BQA_CANON_REG");
+      j->canon_full_columns = c && c[0] == '1';
     }
     long total_nodes = 0, total_bonds = 0;
     for (int c = 0; c < classes; ++c) {
