@@ -453,3 +453,137 @@ __global__ __launch_bounds__(64) void k_mfma8_ext64(ClassView cv, const cplx<R>*
 }
 
 }  // namespace bqa
+
+namespace bqa {
+
+// ---------------------------------------------------------------------------
+// Fused absorb + Z/X gates + symmetric gauge + message re-initialisation,
+// FP32, chi_in = chi_out = 8, degree 3 (src/bqa/backends.py:416-432,
+// state.py:142-156, 219-227).  Absorbing the canonicaliser C_k of bond k with
+// the ZZ split is a mode product with W_k^(p)[c][n] = g0 C_k[c][n] +
+// g1 (1 - 2p) C_k[8 + c][n]; its matrix depends on the physical index p, so
+// each 16-fibre tile (all of one p for degree 3) takes the operand of its p.
+template <int D, int J>
+__device__ __forceinline__ void mp8_p(const float* X, float* Y, const float (&b0)[4], const float (&b1)[4]) {
+  constexpr int st = cpow8(D - 1 - J);
+  constexpr int TILES = Mf8<D>::TILES;
+  static_assert(TILES % 2 == 0, "p-homogeneous tiles need an even tile count");
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int col = i16 & 7, comp = i16 >> 3;
+#pragma unroll
+  for (int t = 0; t < TILES; ++t) {
+    const bool p1 = t >= TILES / 2;
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    const int base = fbase8<D, J>(t * 16 + i16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kr = 4 * s + kq;
+      acc = mfma4(X[2 * (base + (kr & 7) * st) + (kr >> 3)], p1 ? b1[s] : b0[s], acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bb = fbase8<D, J>(t * 16 + 4 * kq + v);
+      Y[2 * (bb + col * st) + comp] = acc[v];
+    }
+  }
+}
+
+template <int D>
+struct Mf8LocalSmem {
+  float T[2 * Mf8<D>::S];
+  float W1[2 * Mf8<D>::S];
+  float W2[2 * Mf8<D>::S];
+  cplx<float> Wt[D][2][64];  // per bond and p: (W^(p))^T, row-major [n][c]
+  float sl[D][8];            // sqrt(lambda) per bond
+  double red[2];
+};
+
+template <int D>
+__global__ __launch_bounds__(64) void k_mfma8_local(ClassView cv, const cplx<float>* __restrict__ Tin,
+                                                    cplx<float>* Tout, const zd* __restrict__ canon,
+                                                    int canon_rows, int canon_cols, double zz_time,
+                                                    double z_time, double x_time,
+                                                    const double* __restrict__ lmbd, int lstride,
+                                                    cplx<float>* msgs_init) {
+  static_assert(D == 3, "degree 3");
+  __shared__ __align__(16) Mf8LocalSmem<D> sm;
+  constexpr int S = Mf8<D>::S, A = Mf8<D>::A;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63;
+  const double cx = cos(x_time), sx = sin(x_time);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const float4* Tg = reinterpret_cast<const float4*>(Tin + (size_t)i * S);
+    float4* Ts = reinterpret_cast<float4*>(sm.T);
+#pragma unroll
+    for (int x = l; x < S / 2; x += 64) Ts[x] = Tg[x];
+    // (W_k^(p))^T[nn][c] = g0 C[c][nn] + g1 z_p C[8 + c][nn]
+    const int nn = l >> 3, c = l & 7;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
+      cplx<double> g0, g1;
+      zz_split(cv.J[k * n + i] * zz_time, g0, g1);
+      const cplx<double> u = g0 * C[c * canon_cols + nn], v = g1 * C[(8 + c) * canon_cols + nn];
+      sm.Wt[k][0][nn * 8 + c] = cast<float>(u + v);
+      sm.Wt[k][1][nn * 8 + c] = cast<float>(u - v);
+      if (l < 8) sm.sl[k][l] = (float)sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + l]);
+    }
+    __syncthreads();
+    float b[D][2][4];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      msg_operand8(sm.Wt[k][0], b[k][0]);
+      msg_operand8(sm.Wt[k][1], b[k][1]);
+    }
+    mp8_p<D, 0>(sm.T, sm.W1, b[0][0], b[0][1]);
+    __syncthreads();
+    mp8_p<D, 1>(sm.W1, sm.W2, b[1][0], b[1][1]);
+    __syncthreads();
+    mp8_p<D, 2>(sm.W2, sm.W1, b[2][0], b[2][1]);
+    __syncthreads();
+    // Z layer, X layer, sqrt(lambda) gauge; accumulate the squared norm
+    const double phi = cv.h[i] * z_time;
+    const double cz = cos(phi), sz = sin(phi);
+    const cplx<float> f0 = mk<float>((float)cz, (float)-sz), f1 = mk<float>((float)cz, (float)sz);
+    cplx<float>* X = reinterpret_cast<cplx<float>*>(sm.W1);
+    double ss = 0.0;
+    for (int a = l; a < A; a += 64) {
+      const cplx<float> t0 = X[a] * f0, t1 = X[A + a] * f1;
+      cplx<float> y0 = mk<float>((float)(cx * t0.x + sx * t1.y), (float)(cx * t0.y - sx * t1.x));
+      cplx<float> y1 = mk<float>((float)(cx * t1.x + sx * t0.y), (float)(cx * t1.y - sx * t0.x));
+      const float f = sm.sl[0][a >> 6] * sm.sl[1][(a >> 3) & 7] * sm.sl[2][a & 7];
+      y0 = scale(y0, f);
+      y1 = scale(y1, f);
+      X[a] = y0;
+      X[A + a] = y1;
+      ss += (double)abs2(y0) + (double)abs2(y1);
+    }
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+    const float inv = (float)(1.0 / sqrt(ss));
+    __syncthreads();
+    float4* To = reinterpret_cast<float4*>(Tout + (size_t)i * S);
+    const float4* Xs = reinterpret_cast<const float4*>(sm.W1);
+#pragma unroll
+    for (int x = l; x < S / 2; x += 64) {
+      float4 v = Xs[x];
+      v.x *= inv;
+      v.y *= inv;
+      v.z *= inv;
+      v.w *= inv;
+      To[x] = v;
+    }
+    // messages diag(lambda) / sum(lambda) at the input positions
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double* lam = lmbd + (size_t)cv.lmbd_pos[k * n + i] * lstride;
+      double tr = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) tr += lam[q];
+      const int a = l >> 3, bcol = l & 7;
+      msgs_init[(size_t)cv.in_pos[k * n + i] * 64 + l] = mk<float>(a == bcol ? (float)(lam[a] / tr) : 0.f, 0.f);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace bqa

@@ -231,7 +231,7 @@ def _run_bloch(cfg, precision, generic):
 
 
 FAST_CASES = {
-    "rr60_chi8": lambda: instances.regular_uniform(60, 3, seed=21, field_sigma=0.4),
+    "rr60_chi8": lambda: instances.regular_qubo(60, 3, seed=21, qmax=10)[:2],
     "grid5x7_chi8": lambda: instances.grid_gaussian(5, 7, seed=22, field_sigma=0.1),
     "heavyhex_chi8": lambda: instances.heavy_hex_gaussian(3, 4, seed=23, field_sigma=0.5),
     "rr40_chi4": lambda: instances.regular_uniform(40, 3, seed=24, field_sigma=0.4),
@@ -298,6 +298,17 @@ def test_fast_kernel_phases_match_generic_fp32(name):
         assert su[0][0] == su[1][0]
         lm = [np.sort(job.get_lambdas().real, axis=1) for job in jobs]
         assert np.abs(lm[0] - lm[1]).max() < 1e-5
+    finally:
+        for job in jobs:
+            job.close()
+    # the fused Trotter step (absorb + gates + gauge kernels) from the same state
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, "single", g) for g in (False, True)]
+    try:
+        for job in jobs:
+            job.run_layer(layers[k]["xtime"], layers[k]["ztime"], ctx_o.pinv_eps, 60, 1e-9, 0.0)
+        r = [job.density()[0] for job in jobs]
+        print(f"[gpu]   fused step |rho fast - generic| {np.abs(r[0] - r[1]).max():.2e}", flush=True)
+        assert np.abs(r[0] - r[1]).max() < 2e-5
     finally:
         for job in jobs:
             job.close()
