@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chi4", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="also run the plain copy kernel on BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -175,6 +177,10 @@ def main():
         raise RuntimeError("no HIP device")
     n = args.qubits
     total_steps = max(BASE_STEPS, args.warmup + args.steps)
+    copy_ms = None
+    if args.copy_calibrate:
+        from bqa_amd.native import copy_calibrate
+        copy_ms = copy_calibrate(args.copy_calibrate, 5)
     log = lambda m: print(m, file=sys.stderr, flush=True)
 
     def one(chi, want_acc):
@@ -239,6 +245,9 @@ def main():
         "accuracy": head.get("accuracy"),
     }
     result.update(extra)
+    if copy_ms is not None:
+        result["copy_calibration"] = {"bytes": args.copy_calibrate, "ms_per_copy": copy_ms,
+                                      "GB_per_s": 2 * args.copy_calibrate / 1e6 / copy_ms}
     print(json.dumps(result))
 
 
