@@ -645,11 +645,11 @@ void local_update(bqa_job* j, int flags, int chi_out, double zz_time, double z_t
                               j->stream)) {
       const int cm = std::max(chi_in, chi_out);
       const long Smax = 2 * ipow_l(cm, c.d);
-      NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 2 * Smax, 0, 2 * Smax, sizeof(cplx<R>));
-      hipLaunchKernelGGL((k_node_local<R>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi_in, chi_out,
-                         Tin, Tout, reinterpret_cast<const zd*>(j->canon.p), n2, ccols, zz_time, flags, z_time,
-                         x_time, reinterpret_cast<const double*>(j->lmbd.p), j->lstride, msg_ptr<R>(j, j->mcur),
-                         reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
+      NodeLaunch L = plan_node(j, c.n, sizeof(cplx<double>) * 2 * Smax, 0, 2 * Smax, sizeof(cplx<double>));
+      hipLaunchKernelGGL((k_node_local<R, double>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi_in,
+                         chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p), n2, ccols, zz_time, flags,
+                         z_time, x_time, reinterpret_cast<const double*>(j->lmbd.p), j->lstride,
+                         msg_ptr<R>(j, j->mcur), reinterpret_cast<cplx<double>*>(L.ws), L.ws_elems);
     }
     LAUNCH_CHECK();
     c.tcur = dst;

@@ -61,8 +61,8 @@ inline bool fast_local_launch<float>(ClassView cv, int chi_in, int chi_out, cons
                                      double zz_time, int flags, double z_time, double x_time, const double* lmbd,
                                      int lstride, cplx<float>* msgs_init, hipStream_t s) {
   if (chi_in != 8 || chi_out != 8 || cv.d != 3 || cv.n == 0 || flags != (kAbsorb | kGates | kGauge)) return false;
-  hipLaunchKernelGGL((k_mfma8_local<3>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, Tin, Tout, canon, rows, cols,
-                     zz_time, z_time, x_time, lmbd, lstride, msgs_init);
+  hipLaunchKernelGGL((k_mfma8_local64<3, float>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, Tin, Tout, canon, rows,
+                     cols, zz_time, z_time, x_time, lmbd, lstride, msgs_init);
   return true;
 }
 

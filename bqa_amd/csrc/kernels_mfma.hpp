@@ -587,3 +587,117 @@ __global__ __launch_bounds__(64) void k_mfma8_local(ClassView cv, const cplx<flo
 }
 
 }  // namespace bqa
+
+namespace bqa {
+
+// FP64-arithmetic variant of k_mfma8_local (tensors stored in R, here FP32):
+// the absorb contraction multiplies by canonicalisers whose entries carry
+// S^{-1/2} factors, and FP32 rounding there is the dominant FP32 error over a
+// full anneal once the extended messages are FP64.
+template <int D, int J>
+__device__ __forceinline__ void mp8d_p(const double* X, double* Y, const double (&b0)[4], const double (&b1)[4]) {
+  constexpr int st = cpow8(D - 1 - J);
+  constexpr int TILES = Mf8<D>::TILES;
+  static_assert(TILES % 2 == 0, "p-homogeneous tiles need an even tile count");
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  const int col = i16 & 7, comp = i16 >> 3;
+#pragma unroll
+  for (int t = 0; t < TILES; ++t) {
+    const bool p1 = t >= TILES / 2;
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    const int base = fbase8<D, J>(t * 16 + i16);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kr = 4 * s + kq;
+      acc = mfma4d(X[2 * (base + (kr & 7) * st) + (kr >> 3)], p1 ? b1[s] : b0[s], acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int bb = fbase8<D, J>(t * 16 + kq + 4 * v);
+      Y[2 * (bb + col * st) + comp] = acc[v];
+    }
+  }
+}
+
+template <int D>
+struct Mf8LocalSmemD {
+  double T[2 * Mf8<D>::S];
+  double W1[2 * Mf8<D>::S];
+  double W2[2 * Mf8<D>::S];
+  cplx<double> Wt[D][2][64];
+  double sl[D][8];
+};
+
+template <int D, typename R>
+__global__ __launch_bounds__(64) void k_mfma8_local64(ClassView cv, const cplx<R>* __restrict__ Tin,
+                                                      cplx<R>* Tout, const zd* __restrict__ canon, int canon_rows,
+                                                      int canon_cols, double zz_time, double z_time, double x_time,
+                                                      const double* __restrict__ lmbd, int lstride,
+                                                      cplx<R>* msgs_init) {
+  static_assert(D == 3, "degree 3");
+  __shared__ __align__(16) Mf8LocalSmemD<D> sm;
+  constexpr int S = Mf8<D>::S, A = Mf8<D>::A;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63;
+  const double cx = cos(x_time), sx = sin(x_time);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<R>* Tg = Tin + (size_t)i * S;
+    cplx<double>* Ts = reinterpret_cast<cplx<double>*>(sm.T);
+    for (int x = l; x < S; x += 64) Ts[x] = cast<double>(Tg[x]);
+    const int nn = l >> 3, c = l & 7;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
+      cplx<double> g0, g1;
+      zz_split(cv.J[k * n + i] * zz_time, g0, g1);
+      const cplx<double> u = g0 * C[c * canon_cols + nn], v = g1 * C[(8 + c) * canon_cols + nn];
+      sm.Wt[k][0][nn * 8 + c] = u + v;
+      sm.Wt[k][1][nn * 8 + c] = u - v;
+      if (l < 8) sm.sl[k][l] = sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + l]);
+    }
+    __syncthreads();
+    double b[D][2][4];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      msg_operand8d(sm.Wt[k][0], b[k][0]);
+      msg_operand8d(sm.Wt[k][1], b[k][1]);
+    }
+    mp8d_p<D, 0>(sm.T, sm.W1, b[0][0], b[0][1]);
+    __syncthreads();
+    mp8d_p<D, 1>(sm.W1, sm.W2, b[1][0], b[1][1]);
+    __syncthreads();
+    mp8d_p<D, 2>(sm.W2, sm.W1, b[2][0], b[2][1]);
+    __syncthreads();
+    const double phi = cv.h[i] * z_time;
+    const double cz = cos(phi), sz = sin(phi);
+    const cplx<double> f0 = mk(cz, -sz), f1 = mk(cz, sz);
+    cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.W1);
+    double ss = 0.0;
+    for (int a = l; a < A; a += 64) {
+      const cplx<double> t0 = X[a] * f0, t1 = X[A + a] * f1;
+      const double f = sm.sl[0][a >> 6] * sm.sl[1][(a >> 3) & 7] * sm.sl[2][a & 7];
+      const cplx<double> y0 = mk(f * (cx * t0.x + sx * t1.y), f * (cx * t0.y - sx * t1.x));
+      const cplx<double> y1 = mk(f * (cx * t1.x + sx * t0.y), f * (cx * t1.y - sx * t0.x));
+      X[a] = y0;
+      X[A + a] = y1;
+      ss += abs2(y0) + abs2(y1);
+    }
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+    const double inv = 1.0 / sqrt(ss);
+    __syncthreads();
+    cplx<R>* To = Tout + (size_t)i * S;
+    for (int x = l; x < S; x += 64) To[x] = cast<R>(scale(X[x], inv));
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double* lam = lmbd + (size_t)cv.lmbd_pos[k * n + i] * lstride;
+      double tr = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) tr += lam[q];
+      const int a = l >> 3, bcol = l & 7;
+      msgs_init[(size_t)cv.in_pos[k * n + i] * 64 + l] = mk<R>(a == bcol ? (R)(lam[a] / tr) : (R)0, (R)0);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace bqa

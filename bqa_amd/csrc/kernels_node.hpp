@@ -250,27 +250,30 @@ enum LocalFlags { kAbsorb = 1, kGates = 2, kGauge = 4 };
 // symmetric gauge with per-node L2 normalisation and message re-initialisation.
 // Tin: (n, 2, chi_in^d) -> Tout: (n, 2, chi_out^d).  Dynamic LDS:
 // 2 * max(S) complex buffers when ws == nullptr.
-template <typename R>
+// R: storage type of tensors and messages; RC: arithmetic type (double for both
+// job precisions: FP32 rounding inside the absorb contraction is amplified by
+// the S^{-1/2} of the canonicalisers).
+template <typename R, typename RC>
 __global__ __launch_bounds__(kNT) void k_node_local(
     ClassView cv, int chi_in, int chi_out, const cplx<R>* __restrict__ Tin, cplx<R>* Tout,
     const cplx<double>* __restrict__ canon, int canon_rows, int canon_cols, double zz_time,
     int flags, double z_time, double x_time, const double* __restrict__ lmbd, int lstride,
-    cplx<R>* msgs_init, cplx<R>* gws, long ws_per_block) {
+    cplx<R>* msgs_init, cplx<RC>* gws, long ws_per_block) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double shred[16];
   const int d = cv.d, n = cv.n;
   const int cmax = chi_in > chi_out ? chi_in : chi_out;
   const int Smax = 2 * ipow(cmax, d);
-  cplx<R>* base = gws ? gws + (size_t)blockIdx.x * ws_per_block : reinterpret_cast<cplx<R>*>(smem_raw);
-  cplx<R>* X = base;
-  cplx<R>* Y = base + Smax;
+  cplx<RC>* base = gws ? gws + (size_t)blockIdx.x * ws_per_block : reinterpret_cast<cplx<RC>*>(smem_raw);
+  cplx<RC>* X = base;
+  cplx<RC>* Y = base + Smax;
   const int S_in = 2 * ipow(chi_in, d);
   const int A_out = ipow(chi_out, d);
   const int S_out = 2 * A_out;
 
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const cplx<R>* Ti = Tin + (size_t)i * S_in;
-    for (int x = threadIdx.x; x < S_in; x += blockDim.x) X[x] = Ti[x];
+    for (int x = threadIdx.x; x < S_in; x += blockDim.x) X[x] = cast<RC>(Ti[x]);
     __syncthreads();
     if (flags & kAbsorb) {
       for (int k = 0; k < d; ++k) {
@@ -288,16 +291,16 @@ __global__ __launch_bounds__(kNT) void k_node_local(
           const int oi = rest / chi_out;
           const double z = (oi / pk) ? -1.0 : 1.0;
           const cplx<double> gz = scale(g1, z);
-          cplx<R> acc = mk<R>(0, 0);
-          const cplx<R>* xs = X + (size_t)oi * chi_in * inner + lo;
+          cplx<RC> acc = mk<RC>(0, 0);
+          const cplx<RC>* xs = X + (size_t)oi * chi_in * inner + lo;
           for (int c = 0; c < chi_in; ++c) {
             const cplx<double> w = g0 * C[c * canon_cols + nk] + gz * C[(chi_in + c) * canon_cols + nk];
-            cfma(acc, xs[c * inner], cast<R>(w));
+            cfma(acc, xs[c * inner], cast<RC>(w));
           }
           Y[y] = acc;
         }
         __syncthreads();
-        cplx<R>* tmp = X;
+        cplx<RC>* tmp = X;
         X = Y;
         Y = tmp;
       }
@@ -306,12 +309,12 @@ __global__ __launch_bounds__(kNT) void k_node_local(
       const double phi = cv.h[i] * z_time;
       const double cz = cos(phi), sz = sin(phi);
       const double cx = cos(x_time), sx = sin(x_time);
-      const cplx<R> f0 = mk<R>((R)cz, (R)-sz), f1 = mk<R>((R)cz, (R)sz);
+      const cplx<RC> f0 = mk<RC>((RC)cz, (RC)-sz), f1 = mk<RC>((RC)cz, (RC)sz);
       for (int a = threadIdx.x; a < A_out; a += blockDim.x) {
-        const cplx<R> t0 = X[a] * f0, t1 = X[A_out + a] * f1;
+        const cplx<RC> t0 = X[a] * f0, t1 = X[A_out + a] * f1;
         // (cos x) t - i (sin x) X t
-        X[a] = mk<R>((R)(cx * t0.x + sx * t1.y), (R)(cx * t0.y - sx * t1.x));
-        X[A_out + a] = mk<R>((R)(cx * t1.x + sx * t0.y), (R)(cx * t1.y - sx * t0.x));
+        X[a] = mk<RC>((RC)(cx * t0.x + sx * t1.y), (RC)(cx * t0.y - sx * t1.x));
+        X[A_out + a] = mk<RC>((RC)(cx * t1.x + sx * t0.y), (RC)(cx * t1.y - sx * t0.x));
       }
       __syncthreads();
     }
@@ -325,12 +328,12 @@ __global__ __launch_bounds__(kNT) void k_node_local(
           rem /= chi_out;
           f *= sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + nk]);
         }
-        const cplx<R> v = scale(X[y], (R)f);
+        const cplx<RC> v = scale(X[y], (RC)f);
         X[y] = v;
         ss += (double)abs2(v);
       }
       ss = block_sum(ss, shred);
-      const R inv = (R)(1.0 / sqrt(ss));
+      const RC inv = (RC)(1.0 / sqrt(ss));
       for (int y = threadIdx.x; y < S_out; y += blockDim.x) X[y] = scale(X[y], inv);
       const int cc = chi_out * chi_out;
       for (int k = 0; k < d; ++k) {
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(kNT) void k_node_local(
       __syncthreads();
     }
     cplx<R>* To = Tout + (size_t)i * S_out;
-    for (int x = threadIdx.x; x < S_out; x += blockDim.x) To[x] = X[x];
+    for (int x = threadIdx.x; x < S_out; x += blockDim.x) To[x] = cast<R>(X[x]);
     __syncthreads();
   }
 }
