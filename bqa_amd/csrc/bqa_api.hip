@@ -129,6 +129,8 @@ struct ClassDev {
   }
 };
 
+enum : unsigned { kGenBP = 1, kGenDensity = 2, kGenExt = 4, kGenCanon = 8, kGenLocal = 16 };
+
 struct bqa_job {
   int prec = BQA_PREC_SINGLE;
   size_t csize = 8;  // bytes per complex element
@@ -150,8 +152,12 @@ struct bqa_job {
   int64_t t_launches = 0;
   double t_ms = 0.0, t_bytes = 0.0;
   std::vector<double> pending_bytes;
-  bool generic_only = false;        // BQA_GENERIC_KERNELS=1: run the generic kernels for every shape
+  // BQA_GENERIC_KERNELS: "1" runs the generic kernels for every shape; otherwise
+  // any of the letters b (BP sweep), d (density), e (extended messages),
+  // c (canonicaliser), l (absorb/gates/gauge) selects them per family.
+  unsigned generic_mask = 0;
   bool canon_full_columns = false;  // BQA_CANON_REG=1: full-column register canonicaliser
+  bool generic(unsigned fam) const { return (generic_mask & fam) != 0; }
 
   bool single() const { return prec == BQA_PREC_SINGLE; }
 };
@@ -393,7 +399,7 @@ void launch_bp_class(bqa_job* j, ClassDev& c) {
     e1 = take_event(j);
     HIPX(hipEventRecord(e0, j->stream));
   }
-  if (j->generic_only || !fast_bp_launch<R>(c.view(), chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
+  if (j->generic(kGenBP) || !fast_bp_launch<R>(c.view(), chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
                          msg_ptr<R>(j, 1), reinterpret_cast<BpStatus*>(j->status.p), j->stream)) {
     const long S = 2 * ipow_l(chi, c.d);
     const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
@@ -530,7 +536,7 @@ void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64
   const int chi = j->chi;
   const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
   cplx<double>* ext = reinterpret_cast<cplx<double>*>(j->ext.p);
-  if (!j->generic_only && chi == 8 && (c.d == 2 || c.d == 3)) {
+  if (!j->generic(kGenExt) && chi == 8 && (c.d == 2 || c.d == 3)) {
     const int grid = std::min(c.n, 4096);
     if (c.d == 3)
       hipLaunchKernelGGL((k_mfma8_ext64<3, R>), dim3(grid), dim3(64), 0, j->stream, c.view(), T,
@@ -566,7 +572,7 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
   // FP64 view of the current messages for the generic extended-message kernel
   const cplx<double>* msgs64 = reinterpret_cast<const cplx<double>*>(msg_ptr<R>(j, j->mcur));
-  bool need_generic = j->generic_only || chi != 8;
+  bool need_generic = j->generic(kGenExt) || chi != 8;
   for (auto& c : j->cls) need_generic |= (c.d != 2 && c.d != 3 && c.d != 0);
   if (sizeof(R) == sizeof(float) && need_generic) {
     const long cnt = (long)j->M * chi * chi;
@@ -587,7 +593,7 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const size_t wsb = canon_ws_bytes(n2);
   const double deps = j->single() ? (double)std::numeric_limits<float>::epsilon()
                                   : std::numeric_limits<double>::epsilon();
-  if (j->generic_only || !canon_reg_dispatch<double>(j, chi, ccols, pinv_eps, deps)) {
+  if (j->generic(kGenCanon) || !canon_reg_dispatch<double>(j, chi, ccols, pinv_eps, deps)) {
     int grid;
     size_t lds;
     unsigned char* ws = nullptr;
@@ -639,7 +645,7 @@ void local_update(bqa_job* j, int flags, int chi_out, double zz_time, double z_t
     ensure_tensor(j, c, dst, chi_out);
     const cplx<R>* Tin = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
     cplx<R>* Tout = reinterpret_cast<cplx<R>*>(c.T[dst].p);
-    if (j->generic_only || !fast_local_launch<R>(c.view(), chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
+    if (j->generic(kGenLocal) || !fast_local_launch<R>(c.view(), chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
                               n2, ccols, zz_time, flags, z_time, x_time,
                               reinterpret_cast<const double*>(j->lmbd.p), j->lstride, msg_ptr<R>(j, j->mcur),
                               j->stream)) {
@@ -676,7 +682,7 @@ void density_impl(bqa_job* j, double* rho_out, double* bloch_out) {
   for (auto& c : j->cls) {
     if (c.n == 0) continue;
     const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
-    if (!j->generic_only && fast_density_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), reinterpret_cast<double*>(j->rho.p),
+    if (!j->generic(kGenDensity) && fast_density_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), reinterpret_cast<double*>(j->rho.p),
                                reinterpret_cast<double*>(j->bloch.p), j->stream)) {
       LAUNCH_CHECK();
       continue;
@@ -823,7 +829,10 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
     HIPX(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
     {
       const char* g = std::getenv("BQA_GENERIC_KERNELS");
-      j->generic_only = g && g[0] == '1';
+      if (g && g[0] == '1') j->generic_mask = ~0u;
+      for (const char* q = g; q && *q; ++q)
+        j->generic_mask |= *q == 'b' ? kGenBP : *q == 'd' ? kGenDensity : *q == 'e' ? kGenExt
+                         : *q == 'c' ? kGenCanon : *q == 'l' ? kGenLocal : 0u;
       const char* c = std::getenv("BQA_CANON_REG");
       j->canon_full_columns = c && c[0] == '1';
     }

@@ -253,14 +253,18 @@ __global__ __launch_bounds__(64) void k_canon_h(const cplx<double>* __restrict__
       top.re[r] = top.im[r] = bot.re[r] = bot.im[r] = 0.0;
     }
     if (active) {
+      // Hermitian part (M + M^H) / 2: the extended message is Hermitian by
+      // construction, and its anti-Hermitian rounding part (from FP32 messages)
+      // is dropped rather than folded into the factorisation.
       const cplx<double>* M = ext + (size_t)(side ? E + edge : edge) * N * N;
 #pragma unroll
       for (int r = 0; r < NH; ++r) {
         const cplx<double> x = M[(r0 + r) * N + ct], y = M[(r0 + r) * N + cb];
-        top.re[r] = x.x;
-        top.im[r] = x.y;
-        bot.re[r] = y.x;
-        bot.im[r] = y.y;
+        const cplx<double> xt = M[ct * N + r0 + r], yt = M[cb * N + r0 + r];
+        top.re[r] = 0.5 * (x.x + xt.x);
+        top.im[r] = 0.5 * (x.y - xt.y);
+        bot.re[r] = 0.5 * (y.x + yt.x);
+        bot.im[r] = 0.5 * (y.y - yt.y);
       }
     }
     hcholesky<N>(top, bot, k, h, base, active);

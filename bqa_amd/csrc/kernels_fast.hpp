@@ -34,10 +34,10 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
   if (cv.d == 3)
     hipLaunchKernelGGL((k_mfma8<3, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
-                       (cplx<float>*)nullptr, 0.0, (double*)nullptr, (double*)nullptr);
+                       (double*)nullptr, (double*)nullptr);
   else
     hipLaunchKernelGGL((k_mfma8<2, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
-                       (cplx<float>*)nullptr, 0.0, (double*)nullptr, (double*)nullptr);
+                       (double*)nullptr, (double*)nullptr);
   return true;
 }
 
@@ -48,10 +48,10 @@ inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>*
   cplx<float>* m = const_cast<cplx<float>*>(msgs);
   if (cv.d == 3)
     hipLaunchKernelGGL((k_mfma8<3, kModeDensity>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
-                       (BpStatus*)nullptr, (cplx<float>*)nullptr, 0.0, rho, bloch);
+                       (BpStatus*)nullptr, rho, bloch);
   else
     hipLaunchKernelGGL((k_mfma8<2, kModeDensity>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
-                       (BpStatus*)nullptr, (cplx<float>*)nullptr, 0.0, rho, bloch);
+                       (BpStatus*)nullptr, rho, bloch);
   return true;
 }
 

@@ -79,42 +79,42 @@ __device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[
   }
 }
 
-// G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J; with PSPLIT, g0
-// sums the p = 0 fibres and g1 the p = 1 fibres.
-template <int D, int J, bool PSPLIT>
-__device__ __forceinline__ void gram8(const float* T, const float* Y, f4v& g0, f4v& g1) {
+// G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J, accumulated in four
+// interleaved MFMA chains (short FP32 sums, independent accumulators) and
+// combined in FP64 into Gs (16 x 16, row-major).  Sequential FP32 accumulation
+// of all fibres in one chain measurably degrades the FP32 BP fixed point.
+template <int D, int J>
+__device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gs) {
   constexpr int st = cpow8(D - 1 - J);
   constexpr int KS = Mf8<D>::KS;
+  static_assert(KS % 4 == 0, "four chains");
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
   const int off = (i16 & 7) * st, comp = i16 >> 3;
-  g0 = f4v{0.f, 0.f, 0.f, 0.f};
-  g1 = f4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 8
-  for (int s = 0; s < KS; ++s) {
-    const int idx = 2 * (fbase8<D, J>(4 * s + kq) + off) + comp;
-    const float a = T[idx], b = Y[idx];
-    if (!PSPLIT || s < KS / 2)
-      g0 = mfma4(a, b, g0);
-    else
-      g1 = mfma4(a, b, g1);
-  }
-}
-
-__device__ __forceinline__ void store_g(float* Gs, const f4v& g) {
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
+  f4v g[4];
 #pragma unroll
-  for (int v = 0; v < 4; ++v) Gs[(4 * kq + v) * 16 + i16] = g[v];
+  for (int q = 0; q < 4; ++q) g[q] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int s = 0; s < KS; s += 4) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int idx = 2 * (fbase8<D, J>(4 * (s + q) + kq) + off) + comp;
+      g[q] = mfma4(T[idx], Y[idx], g[q]);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    Gs[(4 * kq + v) * 16 + i16] = ((double)g[0][v] + (double)g[1][v]) + ((double)g[2][v] + (double)g[3][v]);
 }
 
 // out[c][e] from a 16x16 real Gram stored row-major in Gs.
-__device__ __forceinline__ cplx<float> gram_entry(const float* Gs, int c, int e) {
+__device__ __forceinline__ cplx<double> gram_entry64(const double* Gs, int c, int e) {
   return mk(Gs[c * 16 + e] + Gs[(8 + c) * 16 + 8 + e], Gs[c * 16 + 8 + e] - Gs[(8 + c) * 16 + e]);
 }
 
-__device__ __forceinline__ cplx<float> gram_trace(const float* Gs) {
-  cplx<float> t = mk(0.f, 0.f);
+__device__ __forceinline__ cplx<double> gram_trace64(const double* Gs) {
+  cplx<double> t = mk(0.0, 0.0);
 #pragma unroll
-  for (int c = 0; c < 8; ++c) t = t + gram_entry(Gs, c, c);
+  for (int c = 0; c < 8; ++c) t = t + gram_entry64(Gs, c, c);
   return t;
 }
 
@@ -123,51 +123,33 @@ struct Mf8Smem {
   float T[2 * Mf8<D>::S];
   float W1[2 * Mf8<D>::S];
   float W2[2 * Mf8<D>::S];
-  float G[2][256];
+  double G[256];
 };
 
-// Finish one outgoing message from its Gram: BP mode (trace-normalised chi x chi,
-// dist partials) or extended mode (2chi x 2chi with the ZZ split).
-template <int MODE>
-__device__ __forceinline__ void finish8(float* G0s, float* G1s, const f4v& g0,
-                                        const f4v& g1, long pos, const cplx<float>* min, cplx<float>* mout,
-                                        cplx<float>* ext_out, double theta, double& dnum, double& dden) {
-  const int l = threadIdx.x & 63;
-  store_g(G0s, g0);
-  if (MODE == kModeExt) store_g(G1s, g1);
+// Finish one outgoing BP message from its Gram (in Gs): the Hermitian part
+// (the message is Hermitian by construction), normalised by its real trace in
+// FP64, stored in FP32, with the partial maxima of |new - old| and |new + old|.
+__device__ __forceinline__ void finish8(const double* Gs, long pos, const cplx<float>* min, cplx<float>* mout,
+                                        double& dnum, double& dden) {
+  const int l = threadIdx.x & 63, c = l >> 3, e = l & 7;
   __syncthreads();
-  if (MODE == kModeBP) {
-    const cplx<float> inv = cinv(gram_trace(G0s));
-    const cplx<float> v = gram_entry(G0s, l >> 3, l & 7) * inv;
-    const cplx<float> old = min[pos * 64 + l];
-    dnum = fmax(dnum, (double)cabs(v - old));
-    dden = fmax(dden, (double)cabs(v + old));
-    mout[pos * 64 + l] = v;
-  } else {
-    cplx<double> gd0, gd1;
-    zz_split(theta, gd0, gd1);
-    const cplx<double> trd = cast<double>(gram_trace(G0s) + gram_trace(G1s));
-    const cplx<double> inv = cinv(trd * mk(abs2(gd0) + abs2(gd1), 0.0));
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int o = l + 64 * q;
-      const int row = o >> 4, col = o & 15;
-      const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
-      const cplx<float> a0 = gram_entry(G0s, c, e), a1 = gram_entry(G1s, c, e);
-      const cplx<float> gsum = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
-      const cplx<double> ga = s ? gd1 : gd0, gb = s2 ? gd1 : gd0;
-      ext_out[pos * 256 + o] = cast<float>(conj(ga) * gb * cast<double>(gsum) * inv);
-    }
-  }
+  const double inv = 1.0 / gram_trace64(Gs).x;
+  const cplx<double> a = gram_entry64(Gs, c, e), b = gram_entry64(Gs, e, c);
+  const cplx<float> v = cast<float>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
+  const cplx<float> old = min[pos * 64 + l];
+  dnum = fmax(dnum, (double)cabs(v - old));
+  dden = fmax(dden, (double)cabs(v + old));
+  mout[pos * 64 + l] = v;
   __syncthreads();
 }
 
+// MODE kModeBP: one BP sweep over the class; kModeDensity: rho and Bloch vectors.
 template <int D, int MODE>
 __global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
                                               cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
-                                              cplx<float>* ext_out, double ztime, double* rho_out,
-                                              double* bloch_out) {
+                                              double* rho_out, double* bloch_out) {
   static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(MODE == kModeBP || MODE == kModeDensity, "BP or density");
   __shared__ __align__(16) Mf8Smem<D> sm;
   const int n = cv.n;
   const int l = threadIdx.x & 63;
@@ -240,39 +222,32 @@ __global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* _
       __syncthreads();
       continue;
     }
-    constexpr bool PS = MODE == kModeExt;
-    f4v g0, g1;
     if constexpr (D == 2) {
       mp8<D, 1>(sm.T, sm.W1, bm[1]);          // Y0 = T x1 m1
       __syncthreads();
-      gram8<D, 0, PS>(sm.T, sm.W1, g0, g1);
-      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], min, mout, ext_out,
-                    cv.J[0 * n + i] * ztime, dnum, dden);
+      gram8<D, 0>(sm.T, sm.W1, sm.G);
+      finish8(sm.G, cv.out_pos[0 * n + i], min, mout, dnum, dden);
       mp8<D, 0>(sm.T, sm.W2, bm[0]);          // Y1 = T x0 m0
       __syncthreads();
-      gram8<D, 1, PS>(sm.T, sm.W2, g0, g1);
-      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], min, mout, ext_out,
-                    cv.J[1 * n + i] * ztime, dnum, dden);
+      gram8<D, 1>(sm.T, sm.W2, sm.G);
+      finish8(sm.G, cv.out_pos[1 * n + i], min, mout, dnum, dden);
     } else {
       mp8<D, 2>(sm.T, sm.W1, bm[2]);          // W1 = T x2 m2
       __syncthreads();
       mp8<D, 1>(sm.W1, sm.W2, bm[1]);         // Y0 = W1 x1 m1
       __syncthreads();
-      gram8<D, 0, PS>(sm.T, sm.W2, g0, g1);
-      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], min, mout, ext_out,
-                    cv.J[0 * n + i] * ztime, dnum, dden);
+      gram8<D, 0>(sm.T, sm.W2, sm.G);
+      finish8(sm.G, cv.out_pos[0 * n + i], min, mout, dnum, dden);
       mp8<D, 0>(sm.W1, sm.W2, bm[0]);         // Y1 = W1 x0 m0
       __syncthreads();
-      gram8<D, 1, PS>(sm.T, sm.W2, g0, g1);
-      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], min, mout, ext_out,
-                    cv.J[1 * n + i] * ztime, dnum, dden);
+      gram8<D, 1>(sm.T, sm.W2, sm.G);
+      finish8(sm.G, cv.out_pos[1 * n + i], min, mout, dnum, dden);
       mp8<D, 0>(sm.T, sm.W1, bm[0]);          // W1 = T x0 m0
       __syncthreads();
       mp8<D, 1>(sm.W1, sm.W1, bm[1]);         // Y2 = W1 x1 m1 (in place)
       __syncthreads();
-      gram8<D, 2, PS>(sm.T, sm.W1, g0, g1);
-      finish8<MODE>(sm.G[0], sm.G[1], g0, g1, cv.out_pos[2 * n + i], min, mout, ext_out,
-                    cv.J[2 * n + i] * ztime, dnum, dden);
+      gram8<D, 2>(sm.T, sm.W1, sm.G);
+      finish8(sm.G, cv.out_pos[2 * n + i], min, mout, dnum, dden);
     }
   }
   if (MODE == kModeBP) {

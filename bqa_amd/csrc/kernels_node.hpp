@@ -194,11 +194,15 @@ __global__ __launch_bounds__(kNT) void k_node_contract(
       const long pos = cv.out_pos[k * n + i];
       if (MODE == kModeBP) {
         gram<R, false>(bufT, src, chi, sk, S / chi, part, res);
-        cplx<R> tr = mk<R>(0, 0);
-        for (int c = 0; c < chi; ++c) tr = tr + res[c * chi + c];
-        const cplx<R> inv = cinv(tr);
+        // The message is Hermitian by construction: keep the Hermitian part of
+        // the Gram, normalised by its (real) trace.
+        double tr = 0.0;
+        for (int c = 0; c < chi; ++c) tr += (double)res[c * chi + c].x;
+        const double inv = 1.0 / tr;
         for (int o = threadIdx.x; o < cc; o += blockDim.x) {
-          const cplx<R> v = res[o] * inv;
+          const int c = o / chi, e = o % chi;
+          const cplx<double> a = cast<double>(res[o]), b = cast<double>(res[e * chi + c]);
+          const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
           const cplx<R> old = min[pos * cc + o];
           dnum = fmax(dnum, (double)cabs(v - old));
           dden = fmax(dden, (double)cabs(v + old));
