@@ -3,6 +3,8 @@
 // runs the generic kernel of kernels_node.hpp.
 #pragma once
 
+#include <cstdlib>
+
 #include "common.hpp"
 #include "kernels_node.hpp"
 #include "kernels_mfma.hpp"
@@ -27,17 +29,42 @@ bool fast_density_launch(ClassView, int, const cplx<R>*, const cplx<R>*, double*
 
 // FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp).
 inline int mfma8_grid(int n) { return n < 4096 ? n : 4096; }
+// k_mfma8: one block of W waves per node (degree 3: W = 1, 2 or 4; degree 2: W = 1).
+inline int mfma8_waves() {
+  static const int w = [] {
+    const char* e = std::getenv("BQA_MFMA8_WAVES");
+    const int v = e ? std::atoi(e) : 1;
+    return v == 2 || v == 4 ? v : 1;
+  }();
+  return w;
+}
+
+template <int MODE, int W>
+inline void launch_mfma8_3(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
+                           double* rho, double* bloch, hipStream_t s) {
+  hipLaunchKernelGGL((k_mfma8<3, MODE, W>), dim3(mfma8_grid(cv.n)), dim3(64 * W), 0, s, cv, T, mA, mB, st, rho,
+                     bloch);
+}
+
+template <int MODE>
+inline void launch_mfma8(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
+                         double* rho, double* bloch, hipStream_t s) {
+  if (cv.d == 2) {
+    hipLaunchKernelGGL((k_mfma8<2, MODE, 1>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st, rho, bloch);
+    return;
+  }
+  switch (mfma8_waves()) {
+    case 4: launch_mfma8_3<MODE, 4>(cv, T, mA, mB, st, rho, bloch, s); break;
+    case 2: launch_mfma8_3<MODE, 2>(cv, T, mA, mB, st, rho, bloch, s); break;
+    default: launch_mfma8_3<MODE, 1>(cv, T, mA, mB, st, rho, bloch, s); break;
+  }
+}
 
 template <>
 inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB,
                                   BpStatus* st, hipStream_t s) {
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
-  if (cv.d == 3)
-    hipLaunchKernelGGL((k_mfma8<3, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
-                       (double*)nullptr, (double*)nullptr);
-  else
-    hipLaunchKernelGGL((k_mfma8<2, kModeBP>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st,
-                       (double*)nullptr, (double*)nullptr);
+  launch_mfma8<kModeBP>(cv, T, mA, mB, st, nullptr, nullptr, s);
   return true;
 }
 
@@ -46,12 +73,7 @@ inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>*
                                        double* rho, double* bloch, hipStream_t s) {
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
   cplx<float>* m = const_cast<cplx<float>*>(msgs);
-  if (cv.d == 3)
-    hipLaunchKernelGGL((k_mfma8<3, kModeDensity>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
-                       (BpStatus*)nullptr, rho, bloch);
-  else
-    hipLaunchKernelGGL((k_mfma8<2, kModeDensity>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, m, m,
-                       (BpStatus*)nullptr, rho, bloch);
+  launch_mfma8<kModeDensity>(cv, T, m, m, nullptr, rho, bloch, s);
   return true;
 }
 

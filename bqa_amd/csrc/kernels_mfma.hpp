@@ -43,67 +43,164 @@ __device__ __forceinline__ f4v mfma4(float a, float b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// B operand of the real embedding of message m for k-step s (lane-dependent).
-__device__ __forceinline__ void msg_operand8(const cplx<float>* __restrict__ m, float (&bm)[4]) {
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int a = i16 & 7, cn = i16 >> 3;
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int kr = 4 * s + kq;
-    const int b = kr & 7, ck = kr >> 3;
-    const cplx<float> v = m[a * 8 + b];
-    bm[s] = ck == 0 ? (cn == 0 ? v.x : v.y) : (cn == 0 ? -v.y : v.x);
-  }
+// LDS float-index swizzle of the node tensor and work buffers: bit j (j < 6)
+// of a stored index is flipped by the parity of (index & swz_mask<D>(j)).  The
+// map is linear over GF(2), so an address built from disjoint bit fields
+// (lane part | unrolled-loop constant) swizzles as swz(lane) ^ swz(constant),
+// and the constant folds at compile time.  With it every access pattern below
+// (mode-product reads and writes along each bond, Gram reads) touches distinct
+// banks for 32 banks per half-wave and for 64 banks per wave, where the natural
+// layout had up to 8-way conflicts.  The masks never contain bits 0 or 1:
+// complex pairs stay adjacent and each aligned float4 stays one aligned float4
+// with its elements permuted by the low two bits of swz(4 x).
+template <int D>
+__host__ __device__ constexpr int swz_mask(int j) {
+  constexpr int m3[6] = {0x144, 0x124, 0x3C0, 0x490, 0x260, 0x580};
+  constexpr int m2[6] = {0xE8, 0xC0, 0xF0, 0x60, 0xC0, 0x80};
+  return D == 3 ? m3[j] : m2[j];
 }
 
-// Y = X x_J m (float views of complex64 LDS arrays; X may equal Y).
-template <int D, int J>
-__device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[4]) {
-  constexpr int st = cpow8(D - 1 - J);
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int col = i16 & 7, comp = i16 >> 3;
-#pragma unroll
-  for (int t = 0; t < Mf8<D>::TILES; ++t) {
-    f4v acc = {0.f, 0.f, 0.f, 0.f};
-    const int base = fbase8<D, J>(t * 16 + i16);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int kr = 4 * s + kq;
-      acc = mfma4(X[2 * (base + (kr & 7) * st) + (kr >> 3)], bm[s], acc);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int bb = fbase8<D, J>(t * 16 + 4 * kq + v);
-      Y[2 * (bb + col * st) + comp] = acc[v];
-    }
-  }
+template <int D>
+__host__ __device__ constexpr int swz(int w) {
+  int out = w;
+  for (int j = 0; j < 6; ++j) out ^= (__builtin_popcount(swz_mask<D>(j) & w) & 1) << j;
+  return out;
 }
 
-// G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J, accumulated in four
-// interleaved MFMA chains (short FP32 sums, independent accumulators) and
-// combined in FP64 into Gs (16 x 16, row-major).  Sequential FP32 accumulation
-// of all fibres in one chain measurably degrades the FP32 BP fixed point.
-template <int D, int J>
-__device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gs) {
+// Work split of the FP32 BP/density kernel over the W waves of a node's block:
+// each wave takes TPW of the 16-fibre tiles of every mode product and KPW of
+// the k-steps of every Gram.
+template <int D, int W>
+struct Mf8W {
+  static_assert(Mf8<D>::TILES % W == 0, "tiles split evenly over the waves");
+  static constexpr int TPW = Mf8<D>::TILES / W;  // mode-product tiles per wave
+  static constexpr int KPW = Mf8<D>::KS / W;     // Gram k-steps per wave
+  static_assert(KPW % 4 == 0, "four Gram chains per wave");
+};
+
+// Per-lane parts of the swizzled addresses of mp8 and gram8 along each bond,
+// including this wave's fibre offset.  They are computed once per kernel;
+// opaque() hides them from loop-invariant code motion at each use, so the
+// unrolled (lane ^ constant) addresses are formed next to their LDS access
+// instead of being hoisted out of the node loop into hundreds of live registers.
+template <int D>
+struct Swz8Lanes {
+  int rd[D], wr[D], gl[D];
+};
+
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <int D, int W, int J>
+__device__ __forceinline__ void swz8_lanes_j(Swz8Lanes<D>& z) {
   constexpr int st = cpow8(D - 1 - J);
-  constexpr int KS = Mf8<D>::KS;
-  static_assert(KS % 4 == 0, "four chains");
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+  // mode products: this wave's tiles start at fibre 16 TPW w
+  const int fw = swz<D>(2 * fbase8<D, J>(16 * Mf8W<D, W>::TPW * w));
+  // A[i][k]: fibre t*16 + i, element (4 s + kq) & 7 along J, component (4 s + kq) >> 3
+  z.rd[J] = fw ^ swz<D>(2 * (fbase8<D, J>(i16) + kq * st));
+  // D[4 kq + v][i]: fibre t*16 + 4 kq + v, element i & 7 along J, component i >> 3
+  z.wr[J] = fw ^ swz<D>(2 * (fbase8<D, J>(4 * kq) + (i16 & 7) * st) + (i16 >> 3));
+  // Gram operands: fibre 4 s + kq, element i & 7 along J, component i >> 3; this
+  // wave's k-steps start at fibre 4 KPW w
+  z.gl[J] = swz<D>(2 * fbase8<D, J>(4 * Mf8W<D, W>::KPW * w)) ^
+            swz<D>(2 * (fbase8<D, J>(kq) + (i16 & 7) * st) + (i16 >> 3));
+}
+
+template <int D, int W>
+__device__ __forceinline__ Swz8Lanes<D> swz8_lanes() {
+  Swz8Lanes<D> z;
+  swz8_lanes_j<D, W, 0>(z);
+  swz8_lanes_j<D, W, 1>(z);
+  if constexpr (D == 3) swz8_lanes_j<D, W, 2>(z);
+  return z;
+}
+
+// Y = X x_J m on this wave's tiles (float views of swizzled complex64 LDS
+// arrays).  All A operands of the wave's tiles are read first, then the MFMAs
+// run interleaved over the independent tile accumulators.  The waves' tiles
+// are disjoint and every read precedes every write, so X may equal Y.
+template <int D, int W, int J>
+__device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[4], const Swz8Lanes<D>& z) {
+  constexpr int st = cpow8(D - 1 - J);
+  constexpr int TPW = Mf8W<D, W>::TPW;
+  const int rd = opaque(z.rd[J]), wr = opaque(z.wr[J]);
+  float a[TPW][4];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) a[t][s] = X[rd ^ swz<D>(2 * (fbase8<D, J>(t * 16) + 4 * (s & 1) * st) + (s >> 1))];
+  f4v acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = mfma4(a[t][s], bm[s], acc[t]);
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Y[wr ^ swz<D>(2 * fbase8<D, J>(t * 16 + v))] = acc[t][v];
+}
+
+// This wave's part of G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J:
+// four interleaved MFMA chains of short FP32 sums, operands read eight k-steps
+// ahead, combined in FP64 into Gp (16 x 16, row-major).  One sequential FP32
+// chain over all fibres measurably degrades the FP32 BP fixed point.
+template <int D, int W, int J>
+__device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gp, const Swz8Lanes<D>& z) {
+  constexpr int KPW = Mf8W<D, W>::KPW;
+  constexpr int B = KPW < 8 ? KPW : 8;
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int off = (i16 & 7) * st, comp = i16 >> 3;
+  const int gl = opaque(z.gl[J]);
   f4v g[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) g[q] = f4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-  for (int s = 0; s < KS; s += 4) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int idx = 2 * (fbase8<D, J>(4 * (s + q) + kq) + off) + comp;
-      g[q] = mfma4(T[idx], Y[idx], g[q]);
+  for (int s0 = 0; s0 < KPW; s0 += B) {
+    float ta[B], ya[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int idx = gl ^ swz<D>(2 * fbase8<D, J>(4 * (s0 + b)));
+      ta[b] = T[idx];
+      ya[b] = Y[idx];
     }
+#pragma unroll
+    for (int b = 0; b < B; ++b) g[b & 3] = mfma4(ta[b], ya[b], g[b & 3]);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
-    Gs[(4 * kq + v) * 16 + i16] = ((double)g[0][v] + (double)g[1][v]) + ((double)g[2][v] + (double)g[3][v]);
+    Gp[(4 * kq + v) * 16 + i16] = ((double)g[0][v] + (double)g[1][v]) + ((double)g[2][v] + (double)g[3][v]);
+}
+
+// Complex element ci of a swizzled LDS array.
+template <int D>
+__device__ __forceinline__ cplx<float> lds_c(const float* F, int ci) {
+  const int w = swz<D>(2 * ci);
+  return mk(F[w], F[w ^ 1]);
+}
+
+// Stage one node tensor (S complex64, contiguous in HBM) into swizzled LDS with
+// all threads of the block: float4 block x lands on block swz(4 x) >> 2 with
+// element e at position e ^ (swz(4 x) & 3).
+template <int D, int W>
+__device__ __forceinline__ void load_tensor8(const cplx<float>* __restrict__ Tnode, float* Ts) {
+  constexpr int NT = 64 * W;
+  const float4* Tg = reinterpret_cast<const float4*>(Tnode);
+  float4* T4 = reinterpret_cast<float4*>(Ts);
+  const int t = threadIdx.x;
+  const int wl = swz<D>(4 * t);
+#pragma unroll
+  for (int k = 0; k < Mf8<D>::S / 2 / NT; ++k) {
+    const float4 v = Tg[t + NT * k];
+    const int w = wl ^ swz<D>(4 * NT * k);
+    const int xr = w & 3;
+    const float b0 = (xr & 1) ? v.y : v.x, b1 = (xr & 1) ? v.x : v.y;
+    const float b2 = (xr & 1) ? v.w : v.z, b3 = (xr & 1) ? v.z : v.w;
+    T4[w >> 2] = (xr & 2) ? make_float4(b2, b3, b0, b1) : make_float4(b0, b1, b2, b3);
+  }
 }
 
 // out[c][e] from a 16x16 real Gram stored row-major in Gs.
@@ -118,41 +215,73 @@ __device__ __forceinline__ cplx<double> gram_trace64(const double* Gs) {
   return t;
 }
 
-template <int D>
+// Sum of the W per-wave partial Grams, entry (r, c).
+template <int W>
+__device__ __forceinline__ double gsum(const double (*Gp)[256], int r, int c) {
+  double v = Gp[0][r * 16 + c];
+#pragma unroll
+  for (int w = 1; w < W; ++w) v += Gp[w][r * 16 + c];
+  return v;
+}
+
+template <int W>
+__device__ __forceinline__ cplx<double> gram_entry_w(const double (*Gp)[256], int c, int e) {
+  return mk(gsum<W>(Gp, c, e) + gsum<W>(Gp, 8 + c, 8 + e), gsum<W>(Gp, c, 8 + e) - gsum<W>(Gp, 8 + c, e));
+}
+
+template <int D, int W>
 struct Mf8Smem {
   float T[2 * Mf8<D>::S];
   float W1[2 * Mf8<D>::S];
   float W2[2 * Mf8<D>::S];
-  double G[256];
+  double Gp[W][256];
 };
 
-// Finish one outgoing BP message from its Gram (in Gs): the Hermitian part
-// (the message is Hermitian by construction), normalised by its real trace in
-// FP64, stored in FP32, with the partial maxima of |new - old| and |new + old|.
-__device__ __forceinline__ void finish8(const double* Gs, long pos, const cplx<float>* min, cplx<float>* mout,
+// Finish one outgoing BP message from the per-wave Grams (first wave only):
+// the Hermitian part (the message is Hermitian by construction), normalised by
+// its real trace in FP64, stored in FP32 at message position pos, with the
+// partial maxima of |new - old| and |new + old| (old: this lane's entry of the
+// previous message).  Call between the barrier after gram8 and the next one.
+template <int W>
+__device__ __forceinline__ void finish8(const double (*Gp)[256], int pos, cplx<float> old, cplx<float>* mout,
                                         double& dnum, double& dden) {
   const int l = threadIdx.x & 63, c = l >> 3, e = l & 7;
-  __syncthreads();
-  const double inv = 1.0 / gram_trace64(Gs).x;
-  const cplx<double> a = gram_entry64(Gs, c, e), b = gram_entry64(Gs, e, c);
+  double tr = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) tr += gsum<W>(Gp, k, k) + gsum<W>(Gp, 8 + k, 8 + k);
+  const double inv = 1.0 / tr;
+  const cplx<double> a = gram_entry_w<W>(Gp, c, e), b = gram_entry_w<W>(Gp, e, c);
   const cplx<float> v = cast<float>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
-  const cplx<float> old = min[pos * 64 + l];
   dnum = fmax(dnum, (double)cabs(v - old));
   dden = fmax(dden, (double)cabs(v + old));
-  mout[pos * 64 + l] = v;
-  __syncthreads();
+  mout[(size_t)pos * 64 + l] = v;
 }
 
-// MODE kModeBP: one BP sweep over the class; kModeDensity: rho and Bloch vectors.
-template <int D, int MODE>
-__global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
-                                              cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
-                                              double* rho_out, double* bloch_out) {
+// Lane (i16, kq) of the B operand of message m for k-step s reads m[a][b] with
+// a = i16 & 7, b = (4 s + kq) & 7; the real embedding is
+// [[m_r^T, m_i^T], [-m_i^T, m_r^T]].
+__device__ __forceinline__ void msg_operand8(const cplx<float>* __restrict__ m, float (&bm)[4]) {
+  const int l = threadIdx.x & 63, a = l & 7, cn = (l & 15) >> 3, kq = l >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int ck = (4 * s + kq) >> 3;
+    const cplx<float> v = m[a * 8 + ((4 * s + kq) & 7)];
+    bm[s] = ck == 0 ? (cn == 0 ? v.x : v.y) : (cn == 0 ? -v.y : v.x);
+  }
+}
+
+// MODE kModeBP: one BP sweep over the class; kModeDensity: rho and Bloch
+// vectors.  One block of W waves per node.
+template <int D, int MODE, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
+                                               cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
+                                               double* rho_out, double* bloch_out) {
   static_assert(D == 2 || D == 3, "degree 2 or 3");
   static_assert(MODE == kModeBP || MODE == kModeDensity, "BP or density");
-  __shared__ __align__(16) Mf8Smem<D> sm;
+  __shared__ __align__(16) Mf8Smem<D, W> sm;
   const int n = cv.n;
-  const int l = threadIdx.x & 63;
+  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool lead = wave == 0;
   cplx<float>* min = msgA;
   cplx<float>* mout = msgB;
   if (MODE == kModeBP) {
@@ -163,34 +292,41 @@ __global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* _
   }
   double dnum = 0.0, dden = 0.0;
   constexpr int S = Mf8<D>::S;
+  const Swz8Lanes<D> zl = swz8_lanes<D, W>();
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const float4* Tg = reinterpret_cast<const float4*>(T + (size_t)i * S);
-    float4* Ts = reinterpret_cast<float4*>(sm.T);
-#pragma unroll
-    for (int x = l; x < S / 2; x += 64) Ts[x] = Tg[x];
+    load_tensor8<D, W>(T + (size_t)i * S, sm.T);
     float bm[D][4];
 #pragma unroll
     for (int j = 0; j < D; ++j) msg_operand8(min + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
+    int pos[D];
+    cplx<float> old[D];
+    if (MODE == kModeBP && lead) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        pos[k] = cv.out_pos[k * n + i];
+        old[k] = min[(size_t)pos[k] * 64 + l];
+      }
+    }
     __syncthreads();
     if (MODE == kModeDensity) {
-      mp8<D, 0>(sm.T, sm.W1, bm[0]);
+      mp8<D, W, 0>(sm.T, sm.W1, bm[0], zl);
       __syncthreads();
-      mp8<D, 1>(sm.W1, sm.W1, bm[1]);
+      mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);
       __syncthreads();
       if constexpr (D == 3) {
-        mp8<D, 2>(sm.W1, sm.W1, bm[2]);
+        mp8<D, W, 2>(sm.W1, sm.W1, bm[2], zl);
         __syncthreads();
       }
-      // rho[p][q] = sum_a TM[p, a] conj(T[q, a])
+      // rho[p][q] = sum_a TM[p, a] conj(T[q, a]), the a range split over the block
       constexpr int A = Mf8<D>::A;
       cplx<float> r[4] = {mk(0.f, 0.f), mk(0.f, 0.f), mk(0.f, 0.f), mk(0.f, 0.f)};
-      const cplx<float>* TM = reinterpret_cast<const cplx<float>*>(sm.W1);
-      const cplx<float>* Tc = reinterpret_cast<const cplx<float>*>(sm.T);
-      for (int a = l; a < A; a += 64) {
-        cfma_cj(r[0], Tc[a], TM[a]);
-        cfma_cj(r[1], Tc[A + a], TM[a]);
-        cfma_cj(r[2], Tc[a], TM[A + a]);
-        cfma_cj(r[3], Tc[A + a], TM[A + a]);
+      for (int a = threadIdx.x; a < A; a += 64 * W) {
+        const cplx<float> t0 = lds_c<D>(sm.T, a), t1 = lds_c<D>(sm.T, A + a);
+        const cplx<float> m0 = lds_c<D>(sm.W1, a), m1 = lds_c<D>(sm.W1, A + a);
+        cfma_cj(r[0], t0, m0);
+        cfma_cj(r[1], t1, m0);
+        cfma_cj(r[2], t0, m1);
+        cfma_cj(r[3], t1, m1);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -199,13 +335,23 @@ __global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* _
           r[q].y += __shfl_xor(r[q].y, o, 64);
         }
       if (l == 0) {
-        cplx<double> r00 = cast<double>(r[0]), r01 = cast<double>(r[1]);
-        cplx<double> r10 = cast<double>(r[2]), r11 = cast<double>(r[3]);
-        const cplx<double> inv = cinv(r00 + r11);
-        r00 = r00 * inv;
-        r01 = r01 * inv;
-        r10 = r10 * inv;
-        r11 = r11 * inv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          sm.Gp[wave][2 * q] = r[q].x;
+          sm.Gp[wave][2 * q + 1] = r[q].y;
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        cplx<double> rr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          rr[q] = mk(0.0, 0.0);
+#pragma unroll
+          for (int w = 0; w < W; ++w) rr[q] = rr[q] + mk(sm.Gp[w][2 * q], sm.Gp[w][2 * q + 1]);
+        }
+        const cplx<double> inv = cinv(rr[0] + rr[3]);
+        const cplx<double> r00 = rr[0] * inv, r01 = rr[1] * inv, r10 = rr[2] * inv, r11 = rr[3] * inv;
         const long id = cv.node_ids[i];
         if (rho_out) {
           double* ro = rho_out + id * 8;
@@ -222,35 +368,45 @@ __global__ __launch_bounds__(64) void k_mfma8(ClassView cv, const cplx<float>* _
       __syncthreads();
       continue;
     }
+    // Each gram8 is followed by a barrier; the first wave then finishes the
+    // message while the others start the next mode product, whose own barrier
+    // keeps the partial Grams alive until the first wave has read them.
     if constexpr (D == 2) {
-      mp8<D, 1>(sm.T, sm.W1, bm[1]);          // Y0 = T x1 m1
+      mp8<D, W, 1>(sm.T, sm.W1, bm[1], zl);          // Y0 = T x1 m1
       __syncthreads();
-      gram8<D, 0>(sm.T, sm.W1, sm.G);
-      finish8(sm.G, cv.out_pos[0 * n + i], min, mout, dnum, dden);
-      mp8<D, 0>(sm.T, sm.W2, bm[0]);          // Y1 = T x0 m0
+      gram8<D, W, 0>(sm.T, sm.W1, sm.Gp[wave], zl);
       __syncthreads();
-      gram8<D, 1>(sm.T, sm.W2, sm.G);
-      finish8(sm.G, cv.out_pos[1 * n + i], min, mout, dnum, dden);
+      if (lead) finish8<W>(sm.Gp, pos[0], old[0], mout, dnum, dden);
+      mp8<D, W, 0>(sm.T, sm.W2, bm[0], zl);          // Y1 = T x0 m0
+      __syncthreads();
+      gram8<D, W, 1>(sm.T, sm.W2, sm.Gp[wave], zl);
+      __syncthreads();
+      if (lead) finish8<W>(sm.Gp, pos[1], old[1], mout, dnum, dden);
     } else {
-      mp8<D, 2>(sm.T, sm.W1, bm[2]);          // W1 = T x2 m2
+      mp8<D, W, 2>(sm.T, sm.W1, bm[2], zl);          // W1 = T x2 m2
       __syncthreads();
-      mp8<D, 1>(sm.W1, sm.W2, bm[1]);         // Y0 = W1 x1 m1
+      mp8<D, W, 1>(sm.W1, sm.W2, bm[1], zl);         // Y0 = W1 x1 m1
       __syncthreads();
-      gram8<D, 0>(sm.T, sm.W2, sm.G);
-      finish8(sm.G, cv.out_pos[0 * n + i], min, mout, dnum, dden);
-      mp8<D, 0>(sm.W1, sm.W2, bm[0]);         // Y1 = W1 x0 m0
+      gram8<D, W, 0>(sm.T, sm.W2, sm.Gp[wave], zl);
       __syncthreads();
-      gram8<D, 1>(sm.T, sm.W2, sm.G);
-      finish8(sm.G, cv.out_pos[1 * n + i], min, mout, dnum, dden);
-      mp8<D, 0>(sm.T, sm.W1, bm[0]);          // W1 = T x0 m0
+      if (lead) finish8<W>(sm.Gp, pos[0], old[0], mout, dnum, dden);
+      mp8<D, W, 0>(sm.W1, sm.W2, bm[0], zl);         // Y1 = W1 x0 m0
       __syncthreads();
-      mp8<D, 1>(sm.W1, sm.W1, bm[1]);         // Y2 = W1 x1 m1 (in place)
+      gram8<D, W, 1>(sm.T, sm.W2, sm.Gp[wave], zl);
       __syncthreads();
-      gram8<D, 2>(sm.T, sm.W1, sm.G);
-      finish8(sm.G, cv.out_pos[2 * n + i], min, mout, dnum, dden);
+      if (lead) finish8<W>(sm.Gp, pos[1], old[1], mout, dnum, dden);
+      mp8<D, W, 0>(sm.T, sm.W1, bm[0], zl);          // W1 = T x0 m0
+      __syncthreads();
+      mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);         // Y2 = W1 x1 m1 (in place)
+      __syncthreads();
+      gram8<D, W, 2>(sm.T, sm.W1, sm.Gp[wave], zl);
+      __syncthreads();
+      if (lead) finish8<W>(sm.Gp, pos[2], old[2], mout, dnum, dden);
     }
+    // the next node's tensor load overwrites T, which the last Gram read
+    __syncthreads();
   }
-  if (MODE == kModeBP) {
+  if (MODE == kModeBP && lead) {
     for (int o = 32; o > 0; o >>= 1) {
       dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
       dden = fmax(dden, __shfl_xor(dden, o, 64));
@@ -433,142 +589,14 @@ namespace bqa {
 
 // ---------------------------------------------------------------------------
 // Fused absorb + Z/X gates + symmetric gauge + message re-initialisation,
-// FP32, chi_in = chi_out = 8, degree 3 (src/bqa/backends.py:416-432,
-// state.py:142-156, 219-227).  Absorbing the canonicaliser C_k of bond k with
-// the ZZ split is a mode product with W_k^(p)[c][n] = g0 C_k[c][n] +
-// g1 (1 - 2p) C_k[8 + c][n]; its matrix depends on the physical index p, so
-// each 16-fibre tile (all of one p for degree 3) takes the operand of its p.
-template <int D, int J>
-__device__ __forceinline__ void mp8_p(const float* X, float* Y, const float (&b0)[4], const float (&b1)[4]) {
-  constexpr int st = cpow8(D - 1 - J);
-  constexpr int TILES = Mf8<D>::TILES;
-  static_assert(TILES % 2 == 0, "p-homogeneous tiles need an even tile count");
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int col = i16 & 7, comp = i16 >> 3;
-#pragma unroll
-  for (int t = 0; t < TILES; ++t) {
-    const bool p1 = t >= TILES / 2;
-    f4v acc = {0.f, 0.f, 0.f, 0.f};
-    const int base = fbase8<D, J>(t * 16 + i16);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int kr = 4 * s + kq;
-      acc = mfma4(X[2 * (base + (kr & 7) * st) + (kr >> 3)], p1 ? b1[s] : b0[s], acc);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int bb = fbase8<D, J>(t * 16 + 4 * kq + v);
-      Y[2 * (bb + col * st) + comp] = acc[v];
-    }
-  }
-}
-
-template <int D>
-struct Mf8LocalSmem {
-  float T[2 * Mf8<D>::S];
-  float W1[2 * Mf8<D>::S];
-  float W2[2 * Mf8<D>::S];
-  cplx<float> Wt[D][2][64];  // per bond and p: (W^(p))^T, row-major [n][c]
-  float sl[D][8];            // sqrt(lambda) per bond
-  double red[2];
-};
-
-template <int D>
-__global__ __launch_bounds__(64) void k_mfma8_local(ClassView cv, const cplx<float>* __restrict__ Tin,
-                                                    cplx<float>* Tout, const zd* __restrict__ canon,
-                                                    int canon_rows, int canon_cols, double zz_time,
-                                                    double z_time, double x_time,
-                                                    const double* __restrict__ lmbd, int lstride,
-                                                    cplx<float>* msgs_init) {
-  static_assert(D == 3, "degree 3");
-  __shared__ __align__(16) Mf8LocalSmem<D> sm;
-  constexpr int S = Mf8<D>::S, A = Mf8<D>::A;
-  const int n = cv.n;
-  const int l = threadIdx.x & 63;
-  const double cx = cos(x_time), sx = sin(x_time);
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    const float4* Tg = reinterpret_cast<const float4*>(Tin + (size_t)i * S);
-    float4* Ts = reinterpret_cast<float4*>(sm.T);
-#pragma unroll
-    for (int x = l; x < S / 2; x += 64) Ts[x] = Tg[x];
-    // (W_k^(p))^T[nn][c] = g0 C[c][nn] + g1 z_p C[8 + c][nn]
-    const int nn = l >> 3, c = l & 7;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
-      cplx<double> g0, g1;
-      zz_split(cv.J[k * n + i] * zz_time, g0, g1);
-      const cplx<double> u = g0 * C[c * canon_cols + nn], v = g1 * C[(8 + c) * canon_cols + nn];
-      sm.Wt[k][0][nn * 8 + c] = cast<float>(u + v);
-      sm.Wt[k][1][nn * 8 + c] = cast<float>(u - v);
-      if (l < 8) sm.sl[k][l] = (float)sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + l]);
-    }
-    __syncthreads();
-    float b[D][2][4];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      msg_operand8(sm.Wt[k][0], b[k][0]);
-      msg_operand8(sm.Wt[k][1], b[k][1]);
-    }
-    mp8_p<D, 0>(sm.T, sm.W1, b[0][0], b[0][1]);
-    __syncthreads();
-    mp8_p<D, 1>(sm.W1, sm.W2, b[1][0], b[1][1]);
-    __syncthreads();
-    mp8_p<D, 2>(sm.W2, sm.W1, b[2][0], b[2][1]);
-    __syncthreads();
-    // Z layer, X layer, sqrt(lambda) gauge; accumulate the squared norm
-    const double phi = cv.h[i] * z_time;
-    const double cz = cos(phi), sz = sin(phi);
-    const cplx<float> f0 = mk<float>((float)cz, (float)-sz), f1 = mk<float>((float)cz, (float)sz);
-    cplx<float>* X = reinterpret_cast<cplx<float>*>(sm.W1);
-    double ss = 0.0;
-    for (int a = l; a < A; a += 64) {
-      const cplx<float> t0 = X[a] * f0, t1 = X[A + a] * f1;
-      cplx<float> y0 = mk<float>((float)(cx * t0.x + sx * t1.y), (float)(cx * t0.y - sx * t1.x));
-      cplx<float> y1 = mk<float>((float)(cx * t1.x + sx * t0.y), (float)(cx * t1.y - sx * t0.x));
-      const float f = sm.sl[0][a >> 6] * sm.sl[1][(a >> 3) & 7] * sm.sl[2][a & 7];
-      y0 = scale(y0, f);
-      y1 = scale(y1, f);
-      X[a] = y0;
-      X[A + a] = y1;
-      ss += (double)abs2(y0) + (double)abs2(y1);
-    }
-    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
-    const float inv = (float)(1.0 / sqrt(ss));
-    __syncthreads();
-    float4* To = reinterpret_cast<float4*>(Tout + (size_t)i * S);
-    const float4* Xs = reinterpret_cast<const float4*>(sm.W1);
-#pragma unroll
-    for (int x = l; x < S / 2; x += 64) {
-      float4 v = Xs[x];
-      v.x *= inv;
-      v.y *= inv;
-      v.z *= inv;
-      v.w *= inv;
-      To[x] = v;
-    }
-    // messages diag(lambda) / sum(lambda) at the input positions
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const double* lam = lmbd + (size_t)cv.lmbd_pos[k * n + i] * lstride;
-      double tr = 0.0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) tr += lam[q];
-      const int a = l >> 3, bcol = l & 7;
-      msgs_init[(size_t)cv.in_pos[k * n + i] * 64 + l] = mk<float>(a == bcol ? (float)(lam[a] / tr) : 0.f, 0.f);
-    }
-    __syncthreads();
-  }
-}
-
-}  // namespace bqa
-
-namespace bqa {
-
-// FP64-arithmetic variant of k_mfma8_local (tensors stored in R, here FP32):
-// the absorb contraction multiplies by canonicalisers whose entries carry
-// S^{-1/2} factors, and FP32 rounding there is the dominant FP32 error over a
-// full anneal once the extended messages are FP64.
+// chi_in = chi_out = 8, degree 3 (src/bqa/backends.py:416-432,
+// state.py:142-156, 219-227), tensors stored in R (FP32 or FP64) and all
+// arithmetic in FP64: the absorb multiplies by canonicalisers whose entries
+// carry S^{-1/2} factors, and FP32 rounding there dominates the FP32 error over
+// a full anneal.  Absorbing the canonicaliser C_k of bond k with the ZZ split
+// is a mode product with W_k^(p)[c][n] = g0 C_k[c][n] + g1 (1 - 2p) C_k[8 + c][n];
+// its matrix depends on the physical index p, so each 16-fibre tile (all of
+// one p for degree 3) takes the operand of its p.
 template <int D, int J>
 __device__ __forceinline__ void mp8d_p(const double* X, double* Y, const double (&b0)[4], const double (&b1)[4]) {
   constexpr int st = cpow8(D - 1 - J);
