@@ -536,14 +536,7 @@ void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64
   const int chi = j->chi;
   const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
   cplx<double>* ext = reinterpret_cast<cplx<double>*>(j->ext.p);
-  if (!j->generic(kGenExt) && chi == 8 && (c.d == 2 || c.d == 3)) {
-    const int grid = std::min(c.n, 4096);
-    if (c.d == 3)
-      hipLaunchKernelGGL((k_mfma8_ext64<3, R>), dim3(grid), dim3(64), 0, j->stream, c.view(), T,
-                         msg_ptr<R>(j, j->mcur), ext, ztime);
-    else
-      hipLaunchKernelGGL((k_mfma8_ext64<2, R>), dim3(grid), dim3(64), 0, j->stream, c.view(), T,
-                         msg_ptr<R>(j, j->mcur), ext, ztime);
+  if (!j->generic(kGenExt) && fast_ext_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), ext, ztime, j->stream)) {
     LAUNCH_CHECK();
     return;
   }
@@ -572,8 +565,10 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
   // FP64 view of the current messages for the generic extended-message kernel
   const cplx<double>* msgs64 = reinterpret_cast<const cplx<double>*>(msg_ptr<R>(j, j->mcur));
-  bool need_generic = j->generic(kGenExt) || chi != 8;
-  for (auto& c : j->cls) need_generic |= (c.d != 2 && c.d != 3 && c.d != 0);
+  bool need_generic = false;
+  for (auto& c : j->cls)
+    need_generic |= c.d != 0 && c.n > 0 &&
+                    (j->generic(kGenExt) || !((chi == 8 && (c.d == 2 || c.d == 3)) || small_shape(c.d, chi)));
   if (sizeof(R) == sizeof(float) && need_generic) {
     const long cnt = (long)j->M * chi * chi;
     j->conv_msgs.ensure((size_t)cnt * sizeof(cplx<double>), j->dev_bytes, j->stream);

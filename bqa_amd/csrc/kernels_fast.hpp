@@ -8,12 +8,16 @@
 #include "common.hpp"
 #include "kernels_node.hpp"
 #include "kernels_mfma.hpp"
+#include "kernels_small.hpp"
 
 namespace bqa {
 
+// Compile-time-shaped one-wave kernels (kernels_small.hpp) for every (d, chi)
+// they serve; the FP32 chi = 8 specialisations below use MFMA instead.
 template <typename R>
-bool fast_bp_launch(ClassView, int, const cplx<R>*, cplx<R>*, cplx<R>*, BpStatus*, hipStream_t) {
-  return false;
+bool fast_bp_launch(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
+                    hipStream_t s) {
+  return launch_small<kModeBP, R, R>(cv, chi, T, mA, mB, st, nullptr, 0.0, nullptr, nullptr, s);
 }
 
 template <typename R>
@@ -23,8 +27,28 @@ bool fast_local_launch(ClassView, int, int, const cplx<R>*, cplx<R>*, const cplx
 }
 
 template <typename R>
-bool fast_density_launch(ClassView, int, const cplx<R>*, const cplx<R>*, double*, double*, hipStream_t) {
-  return false;
+bool fast_density_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, double* rho, double* bloch,
+                         hipStream_t s) {
+  cplx<R>* m = const_cast<cplx<R>*>(msgs);
+  return launch_small<kModeDensity, R, R>(cv, chi, T, m, m, nullptr, nullptr, 0.0, rho, bloch, s);
+}
+
+// Extended messages in FP64 from R storage: MFMA for chi = 8 (degree 2, 3),
+// k_small for the other served shapes; false leaves the generic kernel.
+template <typename R>
+bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext, double ztime,
+                     hipStream_t s) {
+  if (cv.n == 0) return false;
+  if (chi == 8 && (cv.d == 2 || cv.d == 3)) {
+    const int grid = std::min(cv.n, 4096);
+    if (cv.d == 3)
+      hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext, ztime);
+    else
+      hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext, ztime);
+    return true;
+  }
+  cplx<R>* m = const_cast<cplx<R>*>(msgs);
+  return launch_small<kModeExt, R, double>(cv, chi, T, m, m, nullptr, ext, ztime, nullptr, nullptr, s);
 }
 
 // FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp).
@@ -63,7 +87,8 @@ inline void launch_mfma8(ClassView cv, const cplx<float>* T, cplx<float>* mA, cp
 template <>
 inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB,
                                   BpStatus* st, hipStream_t s) {
-  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
+  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
+    return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, 0.0, nullptr, nullptr, s);
   launch_mfma8<kModeBP>(cv, T, mA, mB, st, nullptr, nullptr, s);
   return true;
 }
@@ -71,8 +96,9 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
 template <>
 inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>* T, const cplx<float>* msgs,
                                        double* rho, double* bloch, hipStream_t s) {
-  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3)) return false;
   cplx<float>* m = const_cast<cplx<float>*>(msgs);
+  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
+    return launch_small<kModeDensity, float, float>(cv, chi, T, m, m, nullptr, nullptr, 0.0, rho, bloch, s);
   launch_mfma8<kModeDensity>(cv, T, m, m, nullptr, rho, bloch, s);
   return true;
 }
@@ -83,7 +109,7 @@ inline bool fast_local_launch<float>(ClassView cv, int chi_in, int chi_out, cons
                                      double zz_time, int flags, double z_time, double x_time, const double* lmbd,
                                      int lstride, cplx<float>* msgs_init, hipStream_t s) {
   if (chi_in != 8 || chi_out != 8 || cv.d != 3 || cv.n == 0 || flags != (kAbsorb | kGates | kGauge)) return false;
-  hipLaunchKernelGGL((k_mfma8_local64<3, float>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, Tin, Tout, canon, rows,
+  hipLaunchKernelGGL((k_mfma8_local64<3, 4, float>), dim3(mfma8_grid(cv.n)), dim3(256), 0, s, cv, Tin, Tout, canon, rows,
                      cols, zz_time, z_time, x_time, lmbd, lstride, msgs_init);
   return true;
 }

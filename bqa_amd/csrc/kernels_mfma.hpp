@@ -79,7 +79,9 @@ struct Mf8W {
 };
 
 // Per-lane parts of the swizzled addresses of mp8 and gram8 along each bond,
-// including this wave's fibre offset.  They are computed once per kernel;
+// including this wave's fibre offset, as byte offsets (4 x the float index;
+// the swizzle commutes with the scaling), so an access costs one XOR and the
+// array's LDS base rides in the instruction's offset.  Computed once per kernel;
 // opaque() hides them from loop-invariant code motion at each use, so the
 // unrolled (lane ^ constant) addresses are formed next to their LDS access
 // instead of being hoisted out of the node loop into hundreds of live registers.
@@ -100,13 +102,21 @@ __device__ __forceinline__ void swz8_lanes_j(Swz8Lanes<D>& z) {
   // mode products: this wave's tiles start at fibre 16 TPW w
   const int fw = swz<D>(2 * fbase8<D, J>(16 * Mf8W<D, W>::TPW * w));
   // A[i][k]: fibre t*16 + i, element (4 s + kq) & 7 along J, component (4 s + kq) >> 3
-  z.rd[J] = fw ^ swz<D>(2 * (fbase8<D, J>(i16) + kq * st));
+  z.rd[J] = 4 * (fw ^ swz<D>(2 * (fbase8<D, J>(i16) + kq * st)));
   // D[4 kq + v][i]: fibre t*16 + 4 kq + v, element i & 7 along J, component i >> 3
-  z.wr[J] = fw ^ swz<D>(2 * (fbase8<D, J>(4 * kq) + (i16 & 7) * st) + (i16 >> 3));
+  z.wr[J] = 4 * (fw ^ swz<D>(2 * (fbase8<D, J>(4 * kq) + (i16 & 7) * st) + (i16 >> 3)));
   // Gram operands: fibre 4 s + kq, element i & 7 along J, component i >> 3; this
   // wave's k-steps start at fibre 4 KPW w
-  z.gl[J] = swz<D>(2 * fbase8<D, J>(4 * Mf8W<D, W>::KPW * w)) ^
-            swz<D>(2 * (fbase8<D, J>(kq) + (i16 & 7) * st) + (i16 >> 3));
+  z.gl[J] = 4 * (swz<D>(2 * fbase8<D, J>(4 * Mf8W<D, W>::KPW * w)) ^
+                 swz<D>(2 * (fbase8<D, J>(kq) + (i16 & 7) * st) + (i16 >> 3)));
+}
+
+__device__ __forceinline__ float lds_ld(const float* base, int byte_off) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ void lds_st(float* base, int byte_off, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
 template <int D, int W>
@@ -131,7 +141,8 @@ __device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[
 #pragma unroll
   for (int t = 0; t < TPW; ++t)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) a[t][s] = X[rd ^ swz<D>(2 * (fbase8<D, J>(t * 16) + 4 * (s & 1) * st) + (s >> 1))];
+    for (int s = 0; s < 4; ++s)
+      a[t][s] = lds_ld(X, rd ^ (4 * swz<D>(2 * (fbase8<D, J>(t * 16) + 4 * (s & 1) * st) + (s >> 1))));
   f4v acc[TPW];
 #pragma unroll
   for (int t = 0; t < TPW; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
@@ -142,7 +153,7 @@ __device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[
 #pragma unroll
   for (int t = 0; t < TPW; ++t)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Y[wr ^ swz<D>(2 * fbase8<D, J>(t * 16 + v))] = acc[t][v];
+    for (int v = 0; v < 4; ++v) lds_st(Y, wr ^ (4 * swz<D>(2 * fbase8<D, J>(t * 16 + v))), acc[t][v]);
 }
 
 // This wave's part of G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J:
@@ -163,9 +174,9 @@ __device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gp
     float ta[B], ya[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-      const int idx = gl ^ swz<D>(2 * fbase8<D, J>(4 * (s0 + b)));
-      ta[b] = T[idx];
-      ya[b] = Y[idx];
+      const int off = gl ^ (4 * swz<D>(2 * fbase8<D, J>(4 * (s0 + b))));
+      ta[b] = lds_ld(T, off);
+      ya[b] = lds_ld(Y, off);
     }
 #pragma unroll
     for (int b = 0; b < B; ++b) g[b & 3] = mfma4(ta[b], ya[b], g[b & 3]);
@@ -437,6 +448,8 @@ __device__ __forceinline__ d4v mfma4d(double a, double b, d4v c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// FP64 16x16x4 MFMA lane layout (checked on gfx950): A[i][k] and B[k][j] as in
+// FP32, D[l / 16 + 4 v][l % 16] in result register v.
 template <typename R>
 __device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, double (&bm)[4]) {
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
@@ -451,134 +464,227 @@ __device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, dou
   }
 }
 
-template <int D, int J>
-__device__ __forceinline__ void mp8d(const double* X, double* Y, const double (&bm)[4]) {
+// Per-lane (and per-wave) parts of the FP64 kernels' LDS indices along each
+// bond; the remaining parts of every index are compile-time constants that
+// add to them (all parts occupy disjoint bits of the natural index), so each
+// access is one base register plus an immediate offset.  Computed once per
+// kernel and passed through opaque() at each use (see Swz8Lanes).
+template <int D>
+struct Lin8Lanes {
+  int rd[D], wr[D], gl[D];
+};
+
+template <int D, int W, int J>
+__device__ __forceinline__ void lin8_lanes_j(Lin8Lanes<D>& z) {
   constexpr int st = cpow8(D - 1 - J);
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int col = i16 & 7, comp = i16 >> 3;
+  constexpr int TPW = Mf8<D>::TILES / W, KPW = Mf8<D>::KS / W;
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+  // A[i][k]: fibre (w TPW + t) 16 + i, element (4 s + kq) & 7 along J, component (4 s + kq) >> 3
+  z.rd[J] = 2 * (fbase8<D, J>(w * TPW * 16 + i16) + kq * st);
+  // D[kq + 4 v][i] (FP64 layout): fibre (w TPW + t) 16 + kq + 4 v, element i & 7, component i >> 3
+  z.wr[J] = 2 * (fbase8<D, J>(w * TPW * 16 + kq) + (i16 & 7) * st) + (i16 >> 3);
+  // Gram operands: fibre 4 (w KPW + s) + kq, element i & 7 along J, component i >> 3
+  z.gl[J] = 2 * (fbase8<D, J>(4 * w * KPW + kq) + (i16 & 7) * st) + (i16 >> 3);
+}
+
+template <int D, int W>
+__device__ __forceinline__ Lin8Lanes<D> lin8_lanes() {
+  Lin8Lanes<D> z;
+  lin8_lanes_j<D, W, 0>(z);
+  lin8_lanes_j<D, W, 1>(z);
+  if constexpr (D == 3) lin8_lanes_j<D, W, 2>(z);
+  return z;
+}
+
+// Y = X x_J m over this wave's TPW tiles (double views of complex128 LDS
+// arrays); with P, tiles in the p = 1 half take the operand b1.  All A operands
+// of the wave's tiles are read before the interleaved MFMAs and every read
+// precedes every write, so X may equal Y (the waves' tiles are disjoint).
+template <int D, int W, int J, bool P>
+__device__ __forceinline__ void mp8dw(const double* X, double* Y, const double (&b0)[4], const double (&b1)[4],
+                                      const Lin8Lanes<D>& z) {
+  constexpr int st = cpow8(D - 1 - J);
+  constexpr int TPW = Mf8<D>::TILES / W;
+  const int w = threadIdx.x >> 6;
+  const int rd = opaque(z.rd[J]), wr = opaque(z.wr[J]);
+  double a[TPW][4];
 #pragma unroll
-  for (int t = 0; t < Mf8<D>::TILES; ++t) {
-    d4v acc = {0.0, 0.0, 0.0, 0.0};
-    const int base = fbase8<D, J>(t * 16 + i16);
+  for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int kr = 4 * s + kq;
-      acc = mfma4d(X[2 * (base + (kr & 7) * st) + (kr >> 3)], bm[s], acc);
+    for (int s = 0; s < 4; ++s) a[tt][s] = X[rd + 2 * (fbase8<D, J>(tt * 16) + 4 * (s & 1) * st) + (s >> 1)];
+  d4v acc[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) acc[tt] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) {
+      const bool p1 = P && (w * TPW + tt) >= Mf8<D>::TILES / 2;
+      acc[tt] = mfma4d(a[tt][s], p1 ? b1[s] : b0[s], acc[tt]);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int bb = fbase8<D, J>(t * 16 + kq + 4 * v);
-      Y[2 * (bb + col * st) + comp] = acc[v];
+  for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Y[wr + 2 * fbase8<D, J>(tt * 16 + 4 * v)] = acc[tt][v];
+}
+
+// This wave's k-steps of the p-resolved Gram along bond J (fibres ordered with
+// p slowest, so k-step s belongs to p = s / (KS / 2)): the p = 0 and p = 1
+// partials, each from two interleaved chains (with W >= 2 each wave's k-steps
+// lie in one half and the p = 1 register partial stays zero).
+template <int D, int W, int J>
+__device__ __forceinline__ void gram8dw(const double* T, const double* Y, d4v& g0, d4v& g1,
+                                        const Lin8Lanes<D>& z) {
+  constexpr int KS = Mf8<D>::KS, KPW = KS / W;
+  constexpr int B = KPW < 8 ? KPW : 8;
+  const int gl = opaque(z.gl[J]);
+  d4v c[2][2] = {{d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}},
+                 {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}}};
+#pragma unroll
+  for (int s0 = 0; s0 < KPW; s0 += B) {
+    double ta[B], ya[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int idx = gl + 2 * fbase8<D, J>(4 * (s0 + b));
+      ta[b] = T[idx];
+      ya[b] = Y[idx];
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int p = W == 1 ? (s0 + b) / (KS / 2) : 0;
+      c[p][b & 1] = mfma4d(ta[b], ya[b], c[p][b & 1]);
+    }
+  }
+  g0 = c[0][0] + c[0][1];
+  g1 = c[1][0] + c[1][1];
+}
+
+// Store this wave's Gram partial(s) (16 x 16, row-major) into Gp, one slot per
+// wave, or slots 0 and 1 (p = 0, 1) for a single wave.
+template <int W>
+__device__ __forceinline__ void store_gd(double* Gp, const d4v& g0, const d4v& g1) {
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    Gp[w * 256 + (kq + 4 * v) * 16 + i16] = g0[v] + g1[v];
+    if (W == 1) {
+      Gp[(kq + 4 * v) * 16 + i16] = g0[v];
+      Gp[256 + (kq + 4 * v) * 16 + i16] = g1[v];
     }
   }
 }
 
-template <int D, int J>
-__device__ __forceinline__ void gram8d(const double* T, const double* Y, d4v& g0, d4v& g1) {
-  constexpr int st = cpow8(D - 1 - J);
-  constexpr int KS = Mf8<D>::KS;
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int off = (i16 & 7) * st, comp = i16 >> 3;
-  g0 = d4v{0.0, 0.0, 0.0, 0.0};
-  g1 = d4v{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-  for (int s = 0; s < KS; ++s) {
-    const int idx = 2 * (fbase8<D, J>(4 * s + kq) + off) + comp;
-    if (s < KS / 2)
-      g0 = mfma4d(T[idx], Y[idx], g0);
-    else
-      g1 = mfma4d(T[idx], Y[idx], g1);
+// Extended message (2chi x 2chi, FP64) at position pos from the partial Grams
+// in Gp, all threads of the block writing.  Partial slots [0, H) hold p = 0 and
+// [H, 2H) hold p = 1 (H = W / 2; one slot each for a single wave); they are
+// first summed into slots 0 and H.
+template <int W>
+__device__ __forceinline__ void finish8_ext64(double* Gp, long pos, cplx<double>* ext_out, double theta) {
+  constexpr int H = W == 1 ? 1 : W / 2;
+  const int t = threadIdx.x, l = t & 63;
+  if constexpr (H > 1) {
+    for (int o = t; o < 256; o += 64 * W) {
+      double a = Gp[o], b = Gp[H * 256 + o];
+#pragma unroll
+      for (int q = 1; q < H; ++q) {
+        a += Gp[q * 256 + o];
+        b += Gp[(H + q) * 256 + o];
+      }
+      Gp[o] = a;
+      Gp[H * 256 + o] = b;
+    }
+    __syncthreads();
+  }
+  const double* G0 = Gp;
+  const double* G1 = Gp + H * 256;
+  // trace over both p blocks: lanes 0..7 of every wave, then a shuffle sum
+  cplx<double> d = mk(0.0, 0.0);
+  if (l < 8) d = gram_entry64(G0, l, l) + gram_entry64(G1, l, l);
+#pragma unroll
+  for (int o = 4; o > 0; o >>= 1) {
+    d.x += __shfl_xor(d.x, o, 64);
+    d.y += __shfl_xor(d.y, o, 64);
+  }
+  const cplx<double> tr = mk(__shfl(d.x, 0, 64), __shfl(d.y, 0, 64));
+  cplx<double> g0c, g1c;
+  zz_split(theta, g0c, g1c);
+  const cplx<double> inv = cinv(tr * mk(abs2(g0c) + abs2(g1c), 0.0));
+  for (int o = t; o < 256; o += 64 * W) {
+    const int row = o >> 4, col = o & 15;
+    const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
+    const cplx<double> a0 = gram_entry64(G0, c, e), a1 = gram_entry64(G1, c, e);
+    const cplx<double> gsum = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
+    const cplx<double> ga = s ? g1c : g0c, gb = s2 ? g1c : g0c;
+    ext_out[pos * 256 + o] = conj(ga) * gb * gsum * inv;
   }
 }
 
-__device__ __forceinline__ void store_gd(double* Gs, const d4v& g) {
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) Gs[(kq + 4 * v) * 16 + i16] = g[v];
-}
-
-__device__ __forceinline__ cplx<double> gram_entry_d(const double* Gs, int c, int e) {
-  return mk(Gs[c * 16 + e] + Gs[(8 + c) * 16 + 8 + e], Gs[c * 16 + 8 + e] - Gs[(8 + c) * 16 + e]);
-}
-
+// Work buffers of the extended-message kernel; for degree 3 the Gram partials
+// (at most 4 x 256 doubles) go into the dead Y buffer, for degree 2 (Y holds 256
+// doubles) into G.
 template <int D>
 struct Mf8SmemD {
   double T[2 * Mf8<D>::S];
   double W1[2 * Mf8<D>::S];
   double W2[2 * Mf8<D>::S];
-  double G[2][256];
+  double G[D == 2 ? 512 : 1];
 };
 
-// Write one 2chi x 2chi extended message (FP64) from the per-p Grams.
-__device__ __forceinline__ void finish8_ext64(double* G0s, double* G1s, const d4v& g0, const d4v& g1, long pos,
-                                              cplx<double>* ext_out, double theta) {
-  const int l = threadIdx.x & 63;
-  store_gd(G0s, g0);
-  store_gd(G1s, g1);
+// One p-resolved Gram and its extended message; Gp receives the partials once
+// every wave has finished reading Y (Gp may be Y).
+template <int D, int W, int J>
+__device__ __forceinline__ void ext_out8(const double* T, const double* Y, double* Gp, long pos,
+                                         cplx<double>* ext_out, double theta, const Lin8Lanes<D>& z) {
+  d4v g0, g1;
+  gram8dw<D, W, J>(T, Y, g0, g1, z);
   __syncthreads();
-  cplx<double> g0c, g1c;
-  zz_split(theta, g0c, g1c);
-  cplx<double> tr = mk(0.0, 0.0);
-#pragma unroll
-  for (int c = 0; c < 8; ++c) tr = tr + gram_entry_d(G0s, c, c) + gram_entry_d(G1s, c, c);
-  const cplx<double> inv = cinv(tr * mk(abs2(g0c) + abs2(g1c), 0.0));
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int o = l + 64 * q;
-    const int row = o >> 4, col = o & 15;
-    const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
-    const cplx<double> a0 = gram_entry_d(G0s, c, e), a1 = gram_entry_d(G1s, c, e);
-    const cplx<double> gsum = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
-    const cplx<double> ga = s ? g1c : g0c, gb = s2 ? g1c : g0c;
-    ext_out[pos * 256 + o] = conj(ga) * gb * gsum * inv;
-  }
+  store_gd<W>(Gp, g0, g1);
+  __syncthreads();
+  finish8_ext64<W>(Gp, pos, ext_out, theta);
   __syncthreads();
 }
 
-template <int D, typename R>
-__global__ __launch_bounds__(64) void k_mfma8_ext64(ClassView cv, const cplx<R>* __restrict__ T,
-                                                    const cplx<R>* __restrict__ msgs, cplx<double>* ext_out,
-                                                    double ztime) {
+// Extended messages in FP64 arithmetic (FP32 or FP64 inputs), chi = 8, one
+// block of W waves per node.
+template <int D, int W, typename R>
+__global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx<R>* __restrict__ T,
+                                                        const cplx<R>* __restrict__ msgs, cplx<double>* ext_out,
+                                                        double ztime) {
   static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(Mf8<D>::TILES % W == 0 && (W == 1 || W % 2 == 0), "wave split");
   __shared__ __align__(16) Mf8SmemD<D> sm;
   const int n = cv.n;
-  const int l = threadIdx.x & 63;
   constexpr int S = Mf8<D>::S;
+  const Lin8Lanes<D> zl = lin8_lanes<D, W>();
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const cplx<R>* Tg = T + (size_t)i * S;
     cplx<double>* Ts = reinterpret_cast<cplx<double>*>(sm.T);
-    for (int x = l; x < S; x += 64) Ts[x] = cast<double>(Tg[x]);
+    for (int x = threadIdx.x; x < S; x += 64 * W) Ts[x] = cast<double>(Tg[x]);
     double bm[D][4];
 #pragma unroll
     for (int j = 0; j < D; ++j) msg_operand8d(msgs + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
     __syncthreads();
-    d4v g0, g1;
     if constexpr (D == 2) {
-      mp8d<D, 1>(sm.T, sm.W1, bm[1]);
+      mp8dw<D, W, 1, false>(sm.T, sm.W1, bm[1], bm[1], zl);
       __syncthreads();
-      gram8d<D, 0>(sm.T, sm.W1, g0, g1);
-      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime);
-      mp8d<D, 0>(sm.T, sm.W2, bm[0]);
+      ext_out8<D, W, 0>(sm.T, sm.W1, sm.G, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime, zl);
+      mp8dw<D, W, 0, false>(sm.T, sm.W2, bm[0], bm[0], zl);
       __syncthreads();
-      gram8d<D, 1>(sm.T, sm.W2, g0, g1);
-      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime);
+      ext_out8<D, W, 1>(sm.T, sm.W2, sm.G, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime, zl);
     } else {
-      mp8d<D, 2>(sm.T, sm.W1, bm[2]);
+      mp8dw<D, W, 2, false>(sm.T, sm.W1, bm[2], bm[2], zl);
       __syncthreads();
-      mp8d<D, 1>(sm.W1, sm.W2, bm[1]);
+      mp8dw<D, W, 1, false>(sm.W1, sm.W2, bm[1], bm[1], zl);
       __syncthreads();
-      gram8d<D, 0>(sm.T, sm.W2, g0, g1);
-      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime);
-      mp8d<D, 0>(sm.W1, sm.W2, bm[0]);
+      ext_out8<D, W, 0>(sm.T, sm.W2, sm.W2, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime, zl);
+      mp8dw<D, W, 0, false>(sm.W1, sm.W2, bm[0], bm[0], zl);
       __syncthreads();
-      gram8d<D, 1>(sm.T, sm.W2, g0, g1);
-      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime);
-      mp8d<D, 0>(sm.T, sm.W1, bm[0]);
+      ext_out8<D, W, 1>(sm.T, sm.W2, sm.W2, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime, zl);
+      mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);
       __syncthreads();
-      mp8d<D, 1>(sm.W1, sm.W1, bm[1]);
+      mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);
       __syncthreads();
-      gram8d<D, 2>(sm.T, sm.W1, g0, g1);
-      finish8_ext64(sm.G[0], sm.G[1], g0, g1, cv.out_pos[2 * n + i], ext_out, cv.J[2 * n + i] * ztime);
+      ext_out8<D, W, 2>(sm.T, sm.W1, sm.W1, cv.out_pos[2 * n + i], ext_out, cv.J[2 * n + i] * ztime, zl);
     }
   }
 }
@@ -596,67 +702,46 @@ namespace bqa {
 // a full anneal.  Absorbing the canonicaliser C_k of bond k with the ZZ split
 // is a mode product with W_k^(p)[c][n] = g0 C_k[c][n] + g1 (1 - 2p) C_k[8 + c][n];
 // its matrix depends on the physical index p, so each 16-fibre tile (all of
-// one p for degree 3) takes the operand of its p.
-template <int D, int J>
-__device__ __forceinline__ void mp8d_p(const double* X, double* Y, const double (&b0)[4], const double (&b1)[4]) {
-  constexpr int st = cpow8(D - 1 - J);
-  constexpr int TILES = Mf8<D>::TILES;
-  static_assert(TILES % 2 == 0, "p-homogeneous tiles need an even tile count");
-  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
-  const int col = i16 & 7, comp = i16 >> 3;
-#pragma unroll
-  for (int t = 0; t < TILES; ++t) {
-    const bool p1 = t >= TILES / 2;
-    d4v acc = {0.0, 0.0, 0.0, 0.0};
-    const int base = fbase8<D, J>(t * 16 + i16);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int kr = 4 * s + kq;
-      acc = mfma4d(X[2 * (base + (kr & 7) * st) + (kr >> 3)], p1 ? b1[s] : b0[s], acc);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int bb = fbase8<D, J>(t * 16 + kq + 4 * v);
-      Y[2 * (bb + col * st) + comp] = acc[v];
-    }
-  }
-}
-
+// one p for degree 3) takes the operand of its p.  One block of W waves per
+// node; the tensor is dead after the first mode product, so two buffers
+// suffice: T -> X -> T -> X.
 template <int D>
 struct Mf8LocalSmemD {
   double T[2 * Mf8<D>::S];
-  double W1[2 * Mf8<D>::S];
-  double W2[2 * Mf8<D>::S];
+  double X[2 * Mf8<D>::S];
   cplx<double> Wt[D][2][64];
   double sl[D][8];
+  double red[4];
 };
 
-template <int D, typename R>
-__global__ __launch_bounds__(64) void k_mfma8_local64(ClassView cv, const cplx<R>* __restrict__ Tin,
-                                                      cplx<R>* Tout, const zd* __restrict__ canon, int canon_rows,
-                                                      int canon_cols, double zz_time, double z_time, double x_time,
-                                                      const double* __restrict__ lmbd, int lstride,
-                                                      cplx<R>* msgs_init) {
+template <int D, int W, typename R>
+__global__ __launch_bounds__(64 * W) void k_mfma8_local64(ClassView cv, const cplx<R>* __restrict__ Tin,
+                                                          cplx<R>* Tout, const zd* __restrict__ canon,
+                                                          int canon_rows, int canon_cols, double zz_time,
+                                                          double z_time, double x_time,
+                                                          const double* __restrict__ lmbd, int lstride,
+                                                          cplx<R>* msgs_init) {
   static_assert(D == 3, "degree 3");
   __shared__ __align__(16) Mf8LocalSmemD<D> sm;
-  constexpr int S = Mf8<D>::S, A = Mf8<D>::A;
+  constexpr int S = Mf8<D>::S, A = Mf8<D>::A, NT = 64 * W;
   const int n = cv.n;
-  const int l = threadIdx.x & 63;
+  const int t = threadIdx.x, l = t & 63, wave = t >> 6;
   const double cx = cos(x_time), sx = sin(x_time);
+  const Lin8Lanes<D> zl = lin8_lanes<D, W>();
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const cplx<R>* Tg = Tin + (size_t)i * S;
     cplx<double>* Ts = reinterpret_cast<cplx<double>*>(sm.T);
-    for (int x = l; x < S; x += 64) Ts[x] = cast<double>(Tg[x]);
-    const int nn = l >> 3, c = l & 7;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
+    for (int x = t; x < S; x += NT) Ts[x] = cast<double>(Tg[x]);
+    for (int q = t; q < D * 64; q += NT) {
+      const int k = q >> 6, nn = (q >> 3) & 7, c = q & 7;
       const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
       cplx<double> g0, g1;
       zz_split(cv.J[k * n + i] * zz_time, g0, g1);
       const cplx<double> u = g0 * C[c * canon_cols + nn], v = g1 * C[(8 + c) * canon_cols + nn];
       sm.Wt[k][0][nn * 8 + c] = u + v;
       sm.Wt[k][1][nn * 8 + c] = u - v;
-      if (l < 8) sm.sl[k][l] = sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + l]);
+      if (c == 0 && nn == 0)
+        for (int e = 0; e < 8; ++e) sm.sl[k][e] = sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + e]);
     }
     __syncthreads();
     double b[D][2][4];
@@ -665,18 +750,18 @@ __global__ __launch_bounds__(64) void k_mfma8_local64(ClassView cv, const cplx<R
       msg_operand8d(sm.Wt[k][0], b[k][0]);
       msg_operand8d(sm.Wt[k][1], b[k][1]);
     }
-    mp8d_p<D, 0>(sm.T, sm.W1, b[0][0], b[0][1]);
+    mp8dw<D, W, 0, true>(sm.T, sm.X, b[0][0], b[0][1], zl);
     __syncthreads();
-    mp8d_p<D, 1>(sm.W1, sm.W2, b[1][0], b[1][1]);
+    mp8dw<D, W, 1, true>(sm.X, sm.T, b[1][0], b[1][1], zl);
     __syncthreads();
-    mp8d_p<D, 2>(sm.W2, sm.W1, b[2][0], b[2][1]);
+    mp8dw<D, W, 2, true>(sm.T, sm.X, b[2][0], b[2][1], zl);
     __syncthreads();
     const double phi = cv.h[i] * z_time;
     const double cz = cos(phi), sz = sin(phi);
     const cplx<double> f0 = mk(cz, -sz), f1 = mk(cz, sz);
-    cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.W1);
+    cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.X);
     double ss = 0.0;
-    for (int a = l; a < A; a += 64) {
+    for (int a = t; a < A; a += NT) {
       const cplx<double> t0 = X[a] * f0, t1 = X[A + a] * f1;
       const double f = sm.sl[0][a >> 6] * sm.sl[1][(a >> 3) & 7] * sm.sl[2][a & 7];
       const cplx<double> y0 = mk(f * (cx * t0.x + sx * t1.y), f * (cx * t0.y - sx * t1.x));
@@ -686,18 +771,21 @@ __global__ __launch_bounds__(64) void k_mfma8_local64(ClassView cv, const cplx<R
       ss += abs2(y0) + abs2(y1);
     }
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
-    const double inv = 1.0 / sqrt(ss);
+    if (l == 0) sm.red[wave] = ss;
     __syncthreads();
-    cplx<R>* To = Tout + (size_t)i * S;
-    for (int x = l; x < S; x += 64) To[x] = cast<R>(scale(X[x], inv));
+    double tot = 0.0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
+    for (int w = 0; w < W; ++w) tot += sm.red[w];
+    const double inv = 1.0 / sqrt(tot);
+    cplx<R>* To = Tout + (size_t)i * S;
+    for (int x = t; x < S; x += NT) To[x] = cast<R>(scale(X[x], inv));
+    for (int q = t; q < D * 64; q += NT) {
+      const int k = q >> 6, a = (q >> 3) & 7, bcol = q & 7;
       const double* lam = lmbd + (size_t)cv.lmbd_pos[k * n + i] * lstride;
       double tr = 0.0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) tr += lam[q];
-      const int a = l >> 3, bcol = l & 7;
-      msgs_init[(size_t)cv.in_pos[k * n + i] * 64 + l] = mk<R>(a == bcol ? (R)(lam[a] / tr) : (R)0, (R)0);
+      for (int e = 0; e < 8; ++e) tr += lam[e];
+      msgs_init[(size_t)cv.in_pos[k * n + i] * 64 + (q & 63)] = mk<R>(a == bcol ? (R)(lam[a] / tr) : (R)0, (R)0);
     }
     __syncthreads();
   }

@@ -1,0 +1,380 @@
+// Node kernels with compile-time degree D and bond dimension CHI (CHI <= 8):
+// one wave per node, the node tensor and two work buffers in LDS, unrolled
+// VALU mode products and lane-split Grams combined with wave shuffles.
+//
+// They replace the generic k_node_contract (one 256-thread workgroup per node,
+// runtime index arithmetic, a barrier around every small step) on the shapes
+// that occur while the bond dimension ramps up to chi_max, on chi_max = 4, and
+// on every FP64 job.  Same contractions and results as k_node_contract:
+//   BP       (state.py:97-124, backends.py:381-408): new messages, Hermitian
+//            part over its real trace, dist partial maxima;
+//   Ext      (state.py:127-139, backends.py:393-404): 2chi x 2chi extended
+//            messages in FP64 from the p-resolved Gram and the ZZ split;
+//   Density  (state.py:77-94, utils.py:23-27): rho and Bloch vectors.
+// R is the storage type; C the arithmetic type (C = R for BP and density,
+// double for the extended messages).
+#pragma once
+
+#include "common.hpp"
+#include "kernels_node.hpp"
+
+namespace bqa {
+
+constexpr int cpow(int b, int e) { return e <= 0 ? 1 : b * cpow(b, e - 1); }
+
+template <int D, int CHI>
+struct Sm {
+  static constexpr int A = cpow(CHI, D);   // chi^d
+  static constexpr int S = 2 * A;          // complex elements per node
+  static constexpr int Q = CHI * CHI;      // Gram entries
+  static constexpr int NG = 64 / Q;        // lane groups splitting the fibres of a Gram
+  static constexpr int RT = S / CHI;       // fibres along one bond
+  static constexpr bool POW2 = (Q & (Q - 1)) == 0;
+};
+
+template <int D, int CHI, typename C>
+struct SmallSmem {
+  cplx<C> T[Sm<D, CHI>::S];
+  cplx<C> X[Sm<D, CHI>::S];
+  cplx<C> Y[Sm<D, CHI>::S];
+  cplx<C> m[D][CHI * CHI];
+  cplx<C> red[2][64];
+};
+
+// Y = X x_J m (m: CHI x CHI in LDS, row a gives output index a along J).
+template <int D, int CHI, int J, typename C>
+__device__ __forceinline__ void smp(const cplx<C>* X, cplx<C>* Y, const cplx<C>* m) {
+  constexpr int S = Sm<D, CHI>::S, st = cpow(CHI, D - 1 - J);
+  const int l = threadIdx.x & 63;
+#pragma unroll 4
+  for (int q = 0; q < (S + 63) / 64; ++q) {
+    const int idx = l + 64 * q;
+    if (S % 64 == 0 || idx < S) {
+      const int a = (idx / st) % CHI;
+      const int base = idx - a * st;
+      cplx<C> acc = mk<C>(0, 0);
+#pragma unroll
+      for (int b = 0; b < CHI; ++b) cfma(acc, m[a * CHI + b], X[base + b * st]);
+      Y[idx] = acc;
+    }
+  }
+}
+
+// Gram along bond K: G[c][e] = sum over fibres r of conj(T[r, c]) Y[r, e], with
+// PSPLIT separate sums over the p = 0 and p = 1 halves of the fibres (p is the
+// slowest index).  Lane l takes entry o = l % Q and the fibres r = g mod NG,
+// g = l / Q; the partials are combined across g.  On return lanes l < Q hold
+// the full sums g[0] (and g[1] with PSPLIT).
+template <int D, int CHI, int K, bool PSPLIT, typename C>
+__device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<C> (&red)[2][64],
+                                      cplx<C> (&g)[2]) {
+  using M = Sm<D, CHI>;
+  constexpr int sk = cpow(CHI, D - 1 - K);
+  const int l = threadIdx.x & 63;
+  const int o = l % M::Q, grp = l / M::Q;
+  const int c = o / CHI, e = o % CHI;
+  g[0] = mk<C>(0, 0);
+  g[1] = mk<C>(0, 0);
+  if (grp < M::NG) {
+#pragma unroll 8
+    for (int r0 = 0; r0 < M::RT; r0 += M::NG) {
+      const int r = r0 + grp;
+      if (M::RT % M::NG == 0 || r < M::RT) {
+        const int base = (r / sk) * CHI * sk + r % sk;
+        if (PSPLIT && r >= M::RT / 2)
+          cfma_cj(g[1], T[base + c * sk], Y[base + e * sk]);
+        else
+          cfma_cj(g[0], T[base + c * sk], Y[base + e * sk]);
+      }
+    }
+  }
+  if constexpr (M::POW2) {
+#pragma unroll
+    for (int off = M::Q; off < 64; off <<= 1) {
+#pragma unroll
+      for (int p = 0; p < (PSPLIT ? 2 : 1); ++p) {
+        g[p].x += __shfl_xor(g[p].x, off, 64);
+        g[p].y += __shfl_xor(g[p].y, off, 64);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < (PSPLIT ? 2 : 1); ++p) red[p][l] = g[p];
+    __syncthreads();
+    if (l < M::Q) {
+#pragma unroll
+      for (int p = 0; p < (PSPLIT ? 2 : 1); ++p) {
+        cplx<C> s = red[p][l];
+#pragma unroll
+        for (int q = 1; q < M::NG; ++q) s = s + red[p][l + q * M::Q];
+        g[p] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// New BP message at position pos from the Gram held by lanes l < Q.
+template <int CHI, typename R, typename C>
+__device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)[2][64], long pos,
+                                           const cplx<R>* min, cplx<R>* mout, double& dnum, double& dden) {
+  constexpr int Q = CHI * CHI;
+  const int l = threadIdx.x & 63;
+  if (l < Q) red[0][l] = g[0];
+  __syncthreads();
+  if (l < Q) {
+    const int c = l / CHI, e = l % CHI;
+    double tr = 0.0;
+#pragma unroll
+    for (int k = 0; k < CHI; ++k) tr += (double)red[0][k * CHI + k].x;
+    const double inv = 1.0 / tr;
+    const cplx<double> a = cast<double>(red[0][l]), b = cast<double>(red[0][e * CHI + c]);
+    const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
+    const cplx<R> old = min[pos * Q + l];
+    dnum = fmax(dnum, (double)cabs(v - old));
+    dden = fmax(dden, (double)cabs(v + old));
+    mout[pos * Q + l] = v;
+  }
+  __syncthreads();
+}
+
+// Extended message (2chi x 2chi, FP64) at position pos from the p-resolved Gram
+// held by lanes l < Q.
+template <int CHI>
+__device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<double> (&red)[2][64], long pos,
+                                            cplx<double>* ext_out, double theta) {
+  constexpr int Q = CHI * CHI, N2 = 2 * CHI;
+  const int l = threadIdx.x & 63;
+  if (l < Q) {
+    red[0][l] = g[0];
+    red[1][l] = g[1];
+  }
+  __syncthreads();
+  cplx<double> g0c, g1c;
+  zz_split(theta, g0c, g1c);
+  cplx<double> tr = mk(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < CHI; ++k) tr = tr + red[0][k * CHI + k] + red[1][k * CHI + k];
+  const cplx<double> inv = cinv(tr * mk(abs2(g0c) + abs2(g1c), 0.0));
+#pragma unroll
+  for (int q = 0; q < (N2 * N2 + 63) / 64; ++q) {
+    const int o = l + 64 * q;
+    if (o < N2 * N2) {
+      const int row = o / N2, col = o % N2;
+      const int s = row / CHI, c = row % CHI, s2 = col / CHI, e = col % CHI;
+      const cplx<double> a0 = red[0][c * CHI + e], a1 = red[1][c * CHI + e];
+      const cplx<double> gsum = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
+      const cplx<double> ga = s ? g1c : g0c, gb = s2 ? g1c : g0c;
+      ext_out[pos * N2 * N2 + o] = conj(ga) * gb * gsum * inv;
+    }
+  }
+  __syncthreads();
+}
+
+// Outgoing message (BP) or extended message (Ext) along bond K from Y_K.
+template <int D, int CHI, int K, int MODE, typename R, typename C>
+__device__ __forceinline__ void sout(SmallSmem<D, CHI, C>& sm, const cplx<C>* Y, const ClassView& cv, int i,
+                                     const cplx<R>* min, cplx<R>* mout, cplx<double>* ext_out, double ztime,
+                                     double& dnum, double& dden) {
+  cplx<C> g[2];
+  sgram<D, CHI, K, MODE == kModeExt, C>(sm.T, Y, sm.red, g);
+  const long pos = cv.out_pos[K * cv.n + i];
+  if constexpr (MODE == kModeBP) {
+    sfinish_bp<CHI, R, C>(g, sm.red, pos, min, mout, dnum, dden);
+  } else {
+    static_assert(sizeof(C) == sizeof(double), "extended messages in FP64");
+    sfinish_ext<CHI>(g, sm.red, pos, ext_out, cv.J[K * cv.n + i] * ztime);
+  }
+}
+
+template <int D, int CHI, int MODE, typename R, typename C>
+__global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
+                                              cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out, double ztime,
+                                              double* rho_out, double* bloch_out) {
+  static_assert(D >= 2 && D <= 4 && CHI >= 2 && CHI <= 8, "shape");
+  using M = Sm<D, CHI>;
+  __shared__ __align__(16) SmallSmem<D, CHI, C> sm;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63;
+  const cplx<R>* min = msgA;
+  cplx<R>* mout = msgB;
+  if (MODE == kModeBP) {
+    if (st->done) return;
+    const int cur = st->cur;
+    min = cur ? msgB : msgA;
+    mout = cur ? msgA : msgB;
+  }
+  double dnum = 0.0, dden = 0.0;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<R>* Tg = T + (size_t)i * M::S;
+#pragma unroll
+    for (int q = 0; q < (M::S + 63) / 64; ++q) {
+      const int x = l + 64 * q;
+      if (M::S % 64 == 0 || x < M::S) sm.T[x] = cast<C>(Tg[x]);
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const cplx<R>* mj = min + (size_t)cv.in_pos[j * n + i] * M::Q;
+      if (l < M::Q) sm.m[j][l] = cast<C>(mj[l]);
+    }
+    __syncthreads();
+    if constexpr (MODE == kModeDensity) {
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
+      __syncthreads();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);
+      __syncthreads();
+      const cplx<C>* Yf = sm.Y;
+      if constexpr (D >= 3) {
+        smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);
+        __syncthreads();
+        Yf = sm.X;
+      }
+      if constexpr (D == 4) {
+        smp<D, CHI, 3, C>(sm.X, sm.Y, sm.m[3]);
+        __syncthreads();
+        Yf = sm.Y;
+      }
+      // rho[p][q] = sum_a Y[p, a] conj(T[q, a])
+      cplx<C> r[4] = {mk<C>(0, 0), mk<C>(0, 0), mk<C>(0, 0), mk<C>(0, 0)};
+      for (int a = l; a < M::A; a += 64) {
+        cfma_cj(r[0], sm.T[a], Yf[a]);
+        cfma_cj(r[1], sm.T[M::A + a], Yf[a]);
+        cfma_cj(r[2], sm.T[a], Yf[M::A + a]);
+        cfma_cj(r[3], sm.T[M::A + a], Yf[M::A + a]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        for (int o = 32; o > 0; o >>= 1) {
+          r[q].x += __shfl_xor(r[q].x, o, 64);
+          r[q].y += __shfl_xor(r[q].y, o, 64);
+        }
+      if (l == 0) {
+        cplx<double> r00 = cast<double>(r[0]), r01 = cast<double>(r[1]);
+        cplx<double> r10 = cast<double>(r[2]), r11 = cast<double>(r[3]);
+        const cplx<double> inv = cinv(r00 + r11);
+        r00 = r00 * inv;
+        r01 = r01 * inv;
+        r10 = r10 * inv;
+        r11 = r11 * inv;
+        const long id = cv.node_ids[i];
+        if (rho_out) {
+          double* ro = rho_out + id * 8;
+          ro[0] = r00.x; ro[1] = r00.y; ro[2] = r01.x; ro[3] = r01.y;
+          ro[4] = r10.x; ro[5] = r10.y; ro[6] = r11.x; ro[7] = r11.y;
+        }
+        if (bloch_out) {
+          double* bo = bloch_out + id * 3;
+          bo[0] = r01.x + r10.x;
+          bo[1] = r10.y - r01.y;
+          bo[2] = r00.x - r11.x;
+        }
+      }
+      __syncthreads();
+      continue;
+    } else if constexpr (D == 2) {
+      smp<D, CHI, 1, C>(sm.T, sm.X, sm.m[1]);  // Y0 = T x1 m1
+      __syncthreads();
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // Y1 = T x0 m0
+      __syncthreads();
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+    } else if constexpr (D == 3) {
+      smp<D, CHI, 2, C>(sm.T, sm.X, sm.m[2]);  // W = T x2 m2
+      __syncthreads();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y0 = W x1 m1
+      __syncthreads();
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      smp<D, CHI, 0, C>(sm.X, sm.Y, sm.m[0]);  // Y1 = W x0 m0
+      __syncthreads();
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // W = T x0 m0
+      __syncthreads();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y2 = W x1 m1
+      __syncthreads();
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, min, mout, ext_out, ztime, dnum, dden);
+    } else {
+      // degree 4: W = T x3 m3 x2 m2 shared by outputs 0 and 1, V = T x0 m0 x1 m1
+      // shared by outputs 2 and 3 (8 mode products instead of 12).
+      smp<D, CHI, 3, C>(sm.T, sm.X, sm.m[3]);
+      __syncthreads();
+      smp<D, CHI, 2, C>(sm.X, sm.Y, sm.m[2]);  // W in Y
+      __syncthreads();
+      smp<D, CHI, 1, C>(sm.Y, sm.X, sm.m[1]);  // Y0 = W x1 m1
+      __syncthreads();
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      smp<D, CHI, 0, C>(sm.Y, sm.X, sm.m[0]);  // Y1 = W x0 m0
+      __syncthreads();
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
+      __syncthreads();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // V in Y
+      __syncthreads();
+      smp<D, CHI, 3, C>(sm.Y, sm.X, sm.m[3]);  // Y2 = V x3 m3
+      __syncthreads();
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);  // Y3 = V x2 m2
+      __syncthreads();
+      sout<D, CHI, 3, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+    }
+  }
+  if (MODE == kModeBP) {
+    for (int o = 32; o > 0; o >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+      dden = fmax(dden, __shfl_xor(dden, o, 64));
+    }
+    if (l == 0) {
+      atomic_max_nonneg(&st->num_bits, dnum);
+      atomic_max_nonneg(&st->den_bits, dden);
+    }
+  }
+}
+
+// Shapes served: degree 2 and 3 with chi 2..8, degree 4 with chi 2..4.
+inline bool small_shape(int d, int chi) {
+  return chi >= 2 && ((d == 2 || d == 3) ? chi <= 8 : (d == 4 && chi <= 4));
+}
+
+inline int small_grid(int n) { return n < 8192 ? n : 8192; }
+
+template <int D, int CHI, int MODE, typename R, typename C>
+inline void launch_small_dc(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
+                            cplx<double>* ext, double ztime, double* rho, double* bloch, hipStream_t s) {
+  hipLaunchKernelGGL((k_small<D, CHI, MODE, R, C>), dim3(small_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st, ext,
+                     ztime, rho, bloch);
+}
+
+template <int D, int MODE, typename R, typename C>
+inline bool launch_small_d(int chi, ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
+                           cplx<double>* ext, double ztime, double* rho, double* bloch, hipStream_t s) {
+  switch (chi) {
+    case 2: launch_small_dc<D, 2, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+    case 3: launch_small_dc<D, 3, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+    case 4: launch_small_dc<D, 4, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+    default: break;
+  }
+  if constexpr (D <= 3) {
+    switch (chi) {
+      case 5: launch_small_dc<D, 5, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+      case 6: launch_small_dc<D, 6, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+      case 7: launch_small_dc<D, 7, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+      case 8: launch_small_dc<D, 8, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+      default: break;
+    }
+  }
+  return false;
+}
+
+// Launch k_small for (d, chi) if the shape is served; false otherwise.
+template <int MODE, typename R, typename C>
+inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
+                         cplx<double>* ext, double ztime, double* rho, double* bloch, hipStream_t s) {
+  if (cv.n == 0 || !small_shape(cv.d, chi)) return false;
+  switch (cv.d) {
+    case 2: return launch_small_d<2, MODE, R, C>(chi, cv, T, mA, mB, st, ext, ztime, rho, bloch, s);
+    case 3: return launch_small_d<3, MODE, R, C>(chi, cv, T, mA, mB, st, ext, ztime, rho, bloch, s);
+    case 4: return launch_small_d<4, MODE, R, C>(chi, cv, T, mA, mB, st, ext, ztime, rho, bloch, s);
+    default: return false;
+  }
+}
+
+}  // namespace bqa
