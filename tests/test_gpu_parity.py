@@ -312,3 +312,52 @@ def test_fast_kernel_phases_match_generic_fp32(name):
     finally:
         for job in jobs:
             job.close()
+
+
+SMALL_CASES = {
+    # (instance, chi): degrees 2-4 on the grid, 1-3 on heavy-hex, 3 on random regular graphs
+    "grid4x5_chi2": (lambda: instances.grid_gaussian(4, 5, seed=31, field_sigma=0.2), 2),
+    "grid4x5_chi3": (lambda: instances.grid_gaussian(4, 5, seed=32, field_sigma=0.2), 3),
+    "grid4x5_chi4": (lambda: instances.grid_gaussian(4, 5, seed=33, field_sigma=0.2), 4),
+    "heavyhex_chi5": (lambda: instances.heavy_hex_gaussian(2, 3, seed=34, field_sigma=0.5), 5),
+    "rr24_chi6": (lambda: instances.regular_uniform(24, 3, seed=35, field_sigma=0.4), 6),
+    "rr24_chi7": (lambda: instances.regular_uniform(24, 3, seed=36, field_sigma=0.4), 7),
+}
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("name", sorted(SMALL_CASES))
+def test_small_kernels_match_generic(name, precision):
+    """Compile-time-shaped kernels (chi < 8, and every FP64 job) against the generic ones on
+    one uploaded state: a BP sweep, the density readout, and the simple update's lambdas
+    (which run the extended-message kernel)."""
+    make, chi = SMALL_CASES[name]
+    nodes, edges = make()
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "bp_eps": 1e-9, "max_bp_iter_number": 60,
+           "schedule": instances.anneal_schedule(4.0, 30)}
+    ctx_o = O.make_context(cfg)
+    st_o = O.init_state(ctx_o)
+    layers = [i for i in ctx_o.instructions if isinstance(i, dict)]
+    k = 0
+    while st_o.chi < chi and k < len(layers) - 1:
+        O.run_layer(ctx_o, st_o, layers[k]["xtime"], layers[k]["ztime"])
+        k += 1
+    print(f"[gpu] small-vs-generic {name} {precision} at step {k}, chi {st_o.chi}", flush=True)
+    assert st_o.chi >= 2
+    tol_m, tol_r, tol_l = (2e-6, 2e-6, 1e-5) if precision == "single" else (1e-12, 1e-12, 1e-10)
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, precision, g) for g in (False, True)]
+    try:
+        m = []
+        for job in jobs:
+            job.run_bp(1, 0.0, 0.0)
+            m.append(job.get_messages())
+        assert np.abs(m[0] - m[1]).max() < tol_m * np.abs(m[1]).max()
+        r = [job.density()[0] for job in jobs]
+        assert np.abs(r[0] - r[1]).max() < tol_r
+        su = [job.simple_update(layers[k]["ztime"], ctx_o.pinv_eps) for job in jobs]
+        assert su[0][0] == su[1][0]
+        lm = [np.sort(job.get_lambdas().real, axis=1) for job in jobs]
+        assert np.abs(lm[0] - lm[1]).max() < tol_l
+    finally:
+        for job in jobs:
+            job.close()
