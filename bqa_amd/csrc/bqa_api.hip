@@ -125,11 +125,24 @@ struct ClassDev {
     v.lmbd_pos = lp;
     v.h = h;
     v.J = J;
+    v.zz = nullptr;
     return v;
   }
 };
 
 enum : unsigned { kGenBP = 1, kGenDensity = 2, kGenExt = 4, kGenCanon = 8, kGenLocal = 16 };
+
+// ZZ-split gates g0, g1 of every edge for one simple update (state.py:127-139,
+// backends.py:519-526): zz[2 e], zz[2 e + 1] = zz_split(J_e ztime).
+__global__ void k_zz_gates(int E, const double* __restrict__ J, double ztime, cplx<double>* zz) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) {
+    cplx<double> g0, g1;
+    zz_split(J[e] * ztime, g0, g1);
+    zz[2 * (size_t)e] = g0;
+    zz[2 * (size_t)e + 1] = g1;
+  }
+}
 
 struct bqa_job {
   int prec = BQA_PREC_SINGLE;
@@ -139,7 +152,8 @@ struct bqa_job {
   int lstride = 1;
   int pred_sweeps = 8;
   std::vector<ClassDev> cls;
-  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws, conv, conv_msgs;
+  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws, conv, conv_msgs, zz;
+  bool zz_valid = false;  // zz holds the gates of the current simple update
   BpStatus* st_host = nullptr;
   double* lmax_host = nullptr;
   hipStream_t stream = nullptr;
@@ -536,7 +550,9 @@ void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64
   const int chi = j->chi;
   const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
   cplx<double>* ext = reinterpret_cast<cplx<double>*>(j->ext.p);
-  if (!j->generic(kGenExt) && fast_ext_launch<R>(c.view(), chi, T, msg_ptr<R>(j, j->mcur), ext, ztime, j->stream)) {
+  ClassView v = c.view();
+  v.zz = reinterpret_cast<const cplx<double>*>(j->zz.p);
+  if (!j->generic(kGenExt) && fast_ext_launch<R>(v, chi, T, msg_ptr<R>(j, j->mcur), ext, j->stream)) {
     LAUNCH_CHECK();
     return;
   }
@@ -563,6 +579,16 @@ template <typename R>
 void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const int chi = j->chi, n2 = 2 * chi;
   j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
+  // the step's ZZ-split gates per edge, read by the extended-message and absorb kernels
+  j->zz.ensure(sizeof(cplx<double>) * 2 * (size_t)std::max(j->E, 1), j->dev_bytes, j->stream);
+  if (j->E > 0) {
+    const double* jv = reinterpret_cast<const double*>(reinterpret_cast<const char*>(j->edges.p) +
+                                                       ((sizeof(int32_t) * 2 * j->E + 15) / 16) * 16);
+    hipLaunchKernelGGL(k_zz_gates, dim3((j->E + 255) / 256), dim3(256), 0, j->stream, j->E, jv, ztime,
+                       reinterpret_cast<cplx<double>*>(j->zz.p));
+    LAUNCH_CHECK();
+  }
+  j->zz_valid = true;
   // FP64 view of the current messages for the generic extended-message kernel
   const cplx<double>* msgs64 = reinterpret_cast<const cplx<double>*>(msg_ptr<R>(j, j->mcur));
   bool need_generic = false;
@@ -640,7 +666,9 @@ void local_update(bqa_job* j, int flags, int chi_out, double zz_time, double z_t
     ensure_tensor(j, c, dst, chi_out);
     const cplx<R>* Tin = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
     cplx<R>* Tout = reinterpret_cast<cplx<R>*>(c.T[dst].p);
-    if (j->generic(kGenLocal) || !fast_local_launch<R>(c.view(), chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
+    ClassView v = c.view();
+    v.zz = (flags & kAbsorb) && j->zz_valid ? reinterpret_cast<const cplx<double>*>(j->zz.p) : nullptr;
+    if (j->generic(kGenLocal) || !fast_local_launch<R>(v, chi_in, chi_out, Tin, Tout, reinterpret_cast<const zd*>(j->canon.p),
                               n2, ccols, zz_time, flags, z_time, x_time,
                               reinterpret_cast<const double*>(j->lmbd.p), j->lstride, msg_ptr<R>(j, j->mcur),
                               j->stream)) {

@@ -17,7 +17,7 @@ namespace bqa {
 template <typename R>
 bool fast_bp_launch(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
                     hipStream_t s) {
-  return launch_small<kModeBP, R, R>(cv, chi, T, mA, mB, st, nullptr, 0.0, nullptr, nullptr, s);
+  return launch_small<kModeBP, R, R>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
 }
 
 template <typename R>
@@ -30,25 +30,26 @@ template <typename R>
 bool fast_density_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, double* rho, double* bloch,
                          hipStream_t s) {
   cplx<R>* m = const_cast<cplx<R>*>(msgs);
-  return launch_small<kModeDensity, R, R>(cv, chi, T, m, m, nullptr, nullptr, 0.0, rho, bloch, s);
+  return launch_small<kModeDensity, R, R>(cv, chi, T, m, m, nullptr, nullptr, rho, bloch, s);
 }
 
 // Extended messages in FP64 from R storage: MFMA for chi = 8 (degree 2, 3),
-// k_small for the other served shapes; false leaves the generic kernel.
+// k_small for the other served shapes; false leaves the generic kernel.  The
+// ZZ-split gates of the step come from cv.zz.
 template <typename R>
-bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext, double ztime,
+bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext,
                      hipStream_t s) {
-  if (cv.n == 0) return false;
+  if (cv.n == 0 || cv.zz == nullptr) return false;
   if (chi == 8 && (cv.d == 2 || cv.d == 3)) {
     const int grid = std::min(cv.n, 4096);
     if (cv.d == 3)
-      hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext, ztime);
+      hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext);
     else
-      hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext, ztime);
+      hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext);
     return true;
   }
   cplx<R>* m = const_cast<cplx<R>*>(msgs);
-  return launch_small<kModeExt, R, double>(cv, chi, T, m, m, nullptr, ext, ztime, nullptr, nullptr, s);
+  return launch_small<kModeExt, R, double>(cv, chi, T, m, m, nullptr, ext, nullptr, nullptr, s);
 }
 
 // FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp).
@@ -88,7 +89,7 @@ template <>
 inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB,
                                   BpStatus* st, hipStream_t s) {
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
-    return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, 0.0, nullptr, nullptr, s);
+    return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
   launch_mfma8<kModeBP>(cv, T, mA, mB, st, nullptr, nullptr, s);
   return true;
 }
@@ -98,7 +99,7 @@ inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>*
                                        double* rho, double* bloch, hipStream_t s) {
   cplx<float>* m = const_cast<cplx<float>*>(msgs);
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
-    return launch_small<kModeDensity, float, float>(cv, chi, T, m, m, nullptr, nullptr, 0.0, rho, bloch, s);
+    return launch_small<kModeDensity, float, float>(cv, chi, T, m, m, nullptr, nullptr, rho, bloch, s);
   launch_mfma8<kModeDensity>(cv, T, m, m, nullptr, rho, bloch, s);
   return true;
 }
@@ -108,7 +109,9 @@ inline bool fast_local_launch<float>(ClassView cv, int chi_in, int chi_out, cons
                                      cplx<float>* Tout, const cplx<double>* canon, int rows, int cols,
                                      double zz_time, int flags, double z_time, double x_time, const double* lmbd,
                                      int lstride, cplx<float>* msgs_init, hipStream_t s) {
-  if (chi_in != 8 || chi_out != 8 || cv.d != 3 || cv.n == 0 || flags != (kAbsorb | kGates | kGauge)) return false;
+  if (chi_in != 8 || chi_out != 8 || cv.d != 3 || cv.n == 0 || flags != (kAbsorb | kGates | kGauge) ||
+      cv.zz == nullptr)
+    return false;
   hipLaunchKernelGGL((k_mfma8_local64<3, 4, float>), dim3(mfma8_grid(cv.n)), dim3(256), 0, s, cv, Tin, Tout, canon, rows,
                      cols, zz_time, z_time, x_time, lmbd, lstride, msgs_init);
   return true;

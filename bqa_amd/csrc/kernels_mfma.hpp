@@ -579,7 +579,8 @@ __device__ __forceinline__ void store_gd(double* Gp, const d4v& g0, const d4v& g
 // [H, 2H) hold p = 1 (H = W / 2; one slot each for a single wave); they are
 // first summed into slots 0 and H.
 template <int W>
-__device__ __forceinline__ void finish8_ext64(double* Gp, long pos, cplx<double>* ext_out, double theta) {
+__device__ __forceinline__ void finish8_ext64(double* Gp, long pos, cplx<double>* ext_out, cplx<double> g0c,
+                                              cplx<double> g1c) {
   constexpr int H = W == 1 ? 1 : W / 2;
   const int t = threadIdx.x, l = t & 63;
   if constexpr (H > 1) {
@@ -606,8 +607,6 @@ __device__ __forceinline__ void finish8_ext64(double* Gp, long pos, cplx<double>
     d.y += __shfl_xor(d.y, o, 64);
   }
   const cplx<double> tr = mk(__shfl(d.x, 0, 64), __shfl(d.y, 0, 64));
-  cplx<double> g0c, g1c;
-  zz_split(theta, g0c, g1c);
   const cplx<double> inv = cinv(tr * mk(abs2(g0c) + abs2(g1c), 0.0));
   for (int o = t; o < 256; o += 64 * W) {
     const int row = o >> 4, col = o & 15;
@@ -632,24 +631,25 @@ struct Mf8SmemD {
 
 // One p-resolved Gram and its extended message; Gp receives the partials once
 // every wave has finished reading Y (Gp may be Y).
+// (edge: the lambda position of the bond, which indexes the ZZ-split gates cv.zz)
 template <int D, int W, int J>
-__device__ __forceinline__ void ext_out8(const double* T, const double* Y, double* Gp, long pos,
-                                         cplx<double>* ext_out, double theta, const Lin8Lanes<D>& z) {
+__device__ __forceinline__ void ext_out8(const double* T, const double* Y, double* Gp, const ClassView& cv,
+                                         int i, cplx<double>* ext_out, const Lin8Lanes<D>& z) {
   d4v g0, g1;
   gram8dw<D, W, J>(T, Y, g0, g1, z);
   __syncthreads();
   store_gd<W>(Gp, g0, g1);
   __syncthreads();
-  finish8_ext64<W>(Gp, pos, ext_out, theta);
+  const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[J * cv.n + i];
+  finish8_ext64<W>(Gp, cv.out_pos[J * cv.n + i], ext_out, zg[0], zg[1]);
   __syncthreads();
 }
 
 // Extended messages in FP64 arithmetic (FP32 or FP64 inputs), chi = 8, one
-// block of W waves per node.
+// block of W waves per node; the ZZ-split gates come from cv.zz.
 template <int D, int W, typename R>
 __global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx<R>* __restrict__ T,
-                                                        const cplx<R>* __restrict__ msgs, cplx<double>* ext_out,
-                                                        double ztime) {
+                                                        const cplx<R>* __restrict__ msgs, cplx<double>* ext_out) {
   static_assert(D == 2 || D == 3, "degree 2 or 3");
   static_assert(Mf8<D>::TILES % W == 0 && (W == 1 || W % 2 == 0), "wave split");
   __shared__ __align__(16) Mf8SmemD<D> sm;
@@ -667,24 +667,24 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx
     if constexpr (D == 2) {
       mp8dw<D, W, 1, false>(sm.T, sm.W1, bm[1], bm[1], zl);
       __syncthreads();
-      ext_out8<D, W, 0>(sm.T, sm.W1, sm.G, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime, zl);
+      ext_out8<D, W, 0>(sm.T, sm.W1, sm.G, cv, i, ext_out, zl);
       mp8dw<D, W, 0, false>(sm.T, sm.W2, bm[0], bm[0], zl);
       __syncthreads();
-      ext_out8<D, W, 1>(sm.T, sm.W2, sm.G, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime, zl);
+      ext_out8<D, W, 1>(sm.T, sm.W2, sm.G, cv, i, ext_out, zl);
     } else {
       mp8dw<D, W, 2, false>(sm.T, sm.W1, bm[2], bm[2], zl);
       __syncthreads();
       mp8dw<D, W, 1, false>(sm.W1, sm.W2, bm[1], bm[1], zl);
       __syncthreads();
-      ext_out8<D, W, 0>(sm.T, sm.W2, sm.W2, cv.out_pos[0 * n + i], ext_out, cv.J[0 * n + i] * ztime, zl);
+      ext_out8<D, W, 0>(sm.T, sm.W2, sm.W2, cv, i, ext_out, zl);
       mp8dw<D, W, 0, false>(sm.W1, sm.W2, bm[0], bm[0], zl);
       __syncthreads();
-      ext_out8<D, W, 1>(sm.T, sm.W2, sm.W2, cv.out_pos[1 * n + i], ext_out, cv.J[1 * n + i] * ztime, zl);
+      ext_out8<D, W, 1>(sm.T, sm.W2, sm.W2, cv, i, ext_out, zl);
       mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);
       __syncthreads();
       mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);
       __syncthreads();
-      ext_out8<D, W, 2>(sm.T, sm.W1, sm.W1, cv.out_pos[2 * n + i], ext_out, cv.J[2 * n + i] * ztime, zl);
+      ext_out8<D, W, 2>(sm.T, sm.W1, sm.W1, cv, i, ext_out, zl);
     }
   }
 }
@@ -712,6 +712,7 @@ struct Mf8LocalSmemD {
   cplx<double> Wt[D][2][64];
   double sl[D][8];
   double red[4];
+  double zf[2];  // cos, sin of the node's Z-gate angle
 };
 
 template <int D, int W, typename R>
@@ -722,6 +723,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_local64(ClassView cv, const cp
                                                           const double* __restrict__ lmbd, int lstride,
                                                           cplx<R>* msgs_init) {
   static_assert(D == 3, "degree 3");
+  (void)zz_time;  // the ZZ-split gates come from cv.zz
   __shared__ __align__(16) Mf8LocalSmemD<D> sm;
   constexpr int S = Mf8<D>::S, A = Mf8<D>::A, NT = 64 * W;
   const int n = cv.n;
@@ -735,13 +737,17 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_local64(ClassView cv, const cp
     for (int q = t; q < D * 64; q += NT) {
       const int k = q >> 6, nn = (q >> 3) & 7, c = q & 7;
       const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
-      cplx<double> g0, g1;
-      zz_split(cv.J[k * n + i] * zz_time, g0, g1);
-      const cplx<double> u = g0 * C[c * canon_cols + nn], v = g1 * C[(8 + c) * canon_cols + nn];
+      const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
+      const cplx<double> u = zg[0] * C[c * canon_cols + nn], v = zg[1] * C[(8 + c) * canon_cols + nn];
       sm.Wt[k][0][nn * 8 + c] = u + v;
       sm.Wt[k][1][nn * 8 + c] = u - v;
       if (c == 0 && nn == 0)
         for (int e = 0; e < 8; ++e) sm.sl[k][e] = sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + e]);
+    }
+    if (t == 0) {
+      const double phi = cv.h[i] * z_time;
+      sm.zf[0] = cos(phi);
+      sm.zf[1] = sin(phi);
     }
     __syncthreads();
     double b[D][2][4];
@@ -756,8 +762,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_local64(ClassView cv, const cp
     __syncthreads();
     mp8dw<D, W, 2, true>(sm.T, sm.X, b[2][0], b[2][1], zl);
     __syncthreads();
-    const double phi = cv.h[i] * z_time;
-    const double cz = cos(phi), sz = sin(phi);
+    const double cz = sm.zf[0], sz = sm.zf[1];
     const cplx<double> f0 = mk(cz, -sz), f1 = mk(cz, sz);
     cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.X);
     double ss = 0.0;

@@ -21,6 +21,9 @@ struct ClassView {
   const int32_t* lmbd_pos;  // [d][n] lambda position of edge (v, w_k)
   const double* h;          // (n)
   const double* J;          // [d][n]
+  // ZZ-split gates of the current simple update per edge (lambda position e):
+  // g0 = zz[2 e], g1 = zz[2 e + 1] (zz_split of J_e ztime); nullptr outside it
+  const cplx<double>* zz;
 };
 
 enum NodeMode { kModeBP = 0, kModeExt = 1, kModeDensity = 2 };

@@ -114,10 +114,11 @@ __device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<C
   }
 }
 
-// New BP message at position pos from the Gram held by lanes l < Q.
+// New BP message at position pos from the Gram held by lanes l < Q (old: this
+// lane's entry of the previous message at pos).
 template <int CHI, typename R, typename C>
-__device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)[2][64], long pos,
-                                           const cplx<R>* min, cplx<R>* mout, double& dnum, double& dden) {
+__device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)[2][64], long pos, cplx<R> old,
+                                           cplx<R>* mout, double& dnum, double& dden) {
   constexpr int Q = CHI * CHI;
   const int l = threadIdx.x & 63;
   if (l < Q) red[0][l] = g[0];
@@ -130,7 +131,6 @@ __device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)
     const double inv = 1.0 / tr;
     const cplx<double> a = cast<double>(red[0][l]), b = cast<double>(red[0][e * CHI + c]);
     const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
-    const cplx<R> old = min[pos * Q + l];
     dnum = fmax(dnum, (double)cabs(v - old));
     dden = fmax(dden, (double)cabs(v + old));
     mout[pos * Q + l] = v;
@@ -139,10 +139,10 @@ __device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)
 }
 
 // Extended message (2chi x 2chi, FP64) at position pos from the p-resolved Gram
-// held by lanes l < Q.
+// held by lanes l < Q and the bond's ZZ-split gates g0c, g1c.
 template <int CHI>
 __device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<double> (&red)[2][64], long pos,
-                                            cplx<double>* ext_out, double theta) {
+                                            cplx<double>* ext_out, cplx<double> g0c, cplx<double> g1c) {
   constexpr int Q = CHI * CHI, N2 = 2 * CHI;
   const int l = threadIdx.x & 63;
   if (l < Q) {
@@ -150,8 +150,6 @@ __device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<dou
     red[1][l] = g[1];
   }
   __syncthreads();
-  cplx<double> g0c, g1c;
-  zz_split(theta, g0c, g1c);
   cplx<double> tr = mk(0.0, 0.0);
 #pragma unroll
   for (int k = 0; k < CHI; ++k) tr = tr + red[0][k * CHI + k] + red[1][k * CHI + k];
@@ -174,22 +172,23 @@ __device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<dou
 // Outgoing message (BP) or extended message (Ext) along bond K from Y_K.
 template <int D, int CHI, int K, int MODE, typename R, typename C>
 __device__ __forceinline__ void sout(SmallSmem<D, CHI, C>& sm, const cplx<C>* Y, const ClassView& cv, int i,
-                                     const cplx<R>* min, cplx<R>* mout, cplx<double>* ext_out, double ztime,
+                                     const cplx<R> (&old)[D], cplx<R>* mout, cplx<double>* ext_out,
                                      double& dnum, double& dden) {
   cplx<C> g[2];
   sgram<D, CHI, K, MODE == kModeExt, C>(sm.T, Y, sm.red, g);
   const long pos = cv.out_pos[K * cv.n + i];
   if constexpr (MODE == kModeBP) {
-    sfinish_bp<CHI, R, C>(g, sm.red, pos, min, mout, dnum, dden);
+    sfinish_bp<CHI, R, C>(g, sm.red, pos, old[K], mout, dnum, dden);
   } else {
     static_assert(sizeof(C) == sizeof(double), "extended messages in FP64");
-    sfinish_ext<CHI>(g, sm.red, pos, ext_out, cv.J[K * cv.n + i] * ztime);
+    const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[K * cv.n + i];
+    sfinish_ext<CHI>(g, sm.red, pos, ext_out, zg[0], zg[1]);
   }
 }
 
 template <int D, int CHI, int MODE, typename R, typename C>
 __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
-                                              cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out, double ztime,
+                                              cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out,
                                               double* rho_out, double* bloch_out) {
   static_assert(D >= 2 && D <= 4 && CHI >= 2 && CHI <= 8, "shape");
   using M = Sm<D, CHI>;
@@ -217,6 +216,11 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
       const cplx<R>* mj = min + (size_t)cv.in_pos[j * n + i] * M::Q;
       if (l < M::Q) sm.m[j][l] = cast<C>(mj[l]);
     }
+    // this lane's entries of the previous outgoing messages, loaded with the inputs
+    cplx<R> old[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      old[k] = (MODE == kModeBP && l < M::Q) ? min[(size_t)cv.out_pos[k * n + i] * M::Q + l] : mk<R>(0, 0);
     __syncthreads();
     if constexpr (MODE == kModeDensity) {
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
@@ -274,24 +278,24 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
     } else if constexpr (D == 2) {
       smp<D, CHI, 1, C>(sm.T, sm.X, sm.m[1]);  // Y0 = T x1 m1
       __syncthreads();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // Y1 = T x0 m0
       __syncthreads();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     } else if constexpr (D == 3) {
       smp<D, CHI, 2, C>(sm.T, sm.X, sm.m[2]);  // W = T x2 m2
       __syncthreads();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y0 = W x1 m1
       __syncthreads();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.X, sm.Y, sm.m[0]);  // Y1 = W x0 m0
       __syncthreads();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // W = T x0 m0
       __syncthreads();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y2 = W x1 m1
       __syncthreads();
-      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
     } else {
       // degree 4: W = T x3 m3 x2 m2 shared by outputs 0 and 1, V = T x0 m0 x1 m1
       // shared by outputs 2 and 3 (8 mode products instead of 12).
@@ -301,20 +305,20 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
       __syncthreads();
       smp<D, CHI, 1, C>(sm.Y, sm.X, sm.m[1]);  // Y0 = W x1 m1
       __syncthreads();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.Y, sm.X, sm.m[0]);  // Y1 = W x0 m0
       __syncthreads();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
       __syncthreads();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // V in Y
       __syncthreads();
       smp<D, CHI, 3, C>(sm.Y, sm.X, sm.m[3]);  // Y2 = V x3 m3
       __syncthreads();
-      sout<D, CHI, 2, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);  // Y3 = V x2 m2
       __syncthreads();
-      sout<D, CHI, 3, MODE, R, C>(sm, sm.X, cv, i, min, mout, ext_out, ztime, dnum, dden);
+      sout<D, CHI, 3, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     }
   }
   if (MODE == kModeBP) {
@@ -338,41 +342,42 @@ inline int small_grid(int n) { return n < 8192 ? n : 8192; }
 
 template <int D, int CHI, int MODE, typename R, typename C>
 inline void launch_small_dc(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                            cplx<double>* ext, double ztime, double* rho, double* bloch, hipStream_t s) {
+                            cplx<double>* ext, double* rho, double* bloch, hipStream_t s) {
   hipLaunchKernelGGL((k_small<D, CHI, MODE, R, C>), dim3(small_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st, ext,
-                     ztime, rho, bloch);
+                     rho, bloch);
 }
 
 template <int D, int MODE, typename R, typename C>
 inline bool launch_small_d(int chi, ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                           cplx<double>* ext, double ztime, double* rho, double* bloch, hipStream_t s) {
+                           cplx<double>* ext, double* rho, double* bloch, hipStream_t s) {
   switch (chi) {
-    case 2: launch_small_dc<D, 2, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
-    case 3: launch_small_dc<D, 3, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
-    case 4: launch_small_dc<D, 4, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+    case 2: launch_small_dc<D, 2, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+    case 3: launch_small_dc<D, 3, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+    case 4: launch_small_dc<D, 4, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
     default: break;
   }
   if constexpr (D <= 3) {
     switch (chi) {
-      case 5: launch_small_dc<D, 5, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
-      case 6: launch_small_dc<D, 6, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
-      case 7: launch_small_dc<D, 7, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
-      case 8: launch_small_dc<D, 8, MODE, R, C>(cv, T, mA, mB, st, ext, ztime, rho, bloch, s); return true;
+      case 5: launch_small_dc<D, 5, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+      case 6: launch_small_dc<D, 6, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+      case 7: launch_small_dc<D, 7, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+      case 8: launch_small_dc<D, 8, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
       default: break;
     }
   }
   return false;
 }
 
-// Launch k_small for (d, chi) if the shape is served; false otherwise.
+// Launch k_small for (d, chi) if the shape is served; false otherwise.  The
+// extended-message mode reads the ZZ-split gates from cv.zz.
 template <int MODE, typename R, typename C>
 inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                         cplx<double>* ext, double ztime, double* rho, double* bloch, hipStream_t s) {
+                         cplx<double>* ext, double* rho, double* bloch, hipStream_t s) {
   if (cv.n == 0 || !small_shape(cv.d, chi)) return false;
   switch (cv.d) {
-    case 2: return launch_small_d<2, MODE, R, C>(chi, cv, T, mA, mB, st, ext, ztime, rho, bloch, s);
-    case 3: return launch_small_d<3, MODE, R, C>(chi, cv, T, mA, mB, st, ext, ztime, rho, bloch, s);
-    case 4: return launch_small_d<4, MODE, R, C>(chi, cv, T, mA, mB, st, ext, ztime, rho, bloch, s);
+    case 2: return launch_small_d<2, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s);
+    case 3: return launch_small_d<3, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s);
+    case 4: return launch_small_d<4, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s);
     default: return false;
   }
 }
