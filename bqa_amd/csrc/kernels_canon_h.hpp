@@ -103,14 +103,20 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
       b = half_sum<N>(b);
       gr = half_sum<N>(gr);
       gi = half_sum<N>(gi);
-      const double ga = sqrt(gr * gr + gi * gi);
-      if (active && fmin(a, b) >= small2 && ga > tol * sqrt(a * b)) {
+      const double g2 = gr * gr + gi * gi;
+      if (active && fmin(a, b) >= small2 && g2 > tol * tol * a * b) {
         rotated = 1;
-        const double zeta = (b - a) / (2.0 * ga);
-        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double c = 1.0 / sqrt(1.0 + t * t);
+        // rotation parameters from reciprocal square roots (c^2 + s^2 = 1 to a few ulp)
+        const double rga = rsqrt(g2);
+        const double zeta = 0.5 * (b - a) * rga;
+        const double w = fabs(zeta) + sqrt(fma(zeta, zeta, 1.0));
+        double rw = __builtin_amdgcn_rcp(w);
+        rw = fma(fma(-w, rw, 1.0), rw, rw);
+        rw = fma(fma(-w, rw, 1.0), rw, rw);
+        const double t = zeta >= 0.0 ? rw : -rw;
+        const double c = rsqrt(fma(t, t, 1.0));
         const double s = c * t;
-        const double er = gr / ga, ei = -gi / ga;
+        const double er = gr * rga, ei = -gi * rga;
 #pragma unroll
         for (int r = 0; r < NH; ++r) {
           const double xr = er * bot.re[r] - ei * bot.im[r];

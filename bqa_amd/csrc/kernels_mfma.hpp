@@ -255,16 +255,19 @@ struct Mf8Smem {
 // previous message).  Call between the barrier after gram8 and the next one.
 template <int W>
 __device__ __forceinline__ void finish8(const double (*Gp)[256], int pos, cplx<float> old, cplx<float>* mout,
-                                        double& dnum, double& dden) {
+                                        double& dnum2, double& dden2) {
   const int l = threadIdx.x & 63, c = l >> 3, e = l & 7;
-  double tr = 0.0;
+  // real trace: lanes 0..7 take one diagonal entry each, then a shuffle sum
+  double d = l < 8 ? gsum<W>(Gp, l, l) + gsum<W>(Gp, 8 + l, 8 + l) : 0.0;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) tr += gsum<W>(Gp, k, k) + gsum<W>(Gp, 8 + k, 8 + k);
-  const double inv = 1.0 / tr;
+  for (int o = 4; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+  const double inv = 1.0 / __shfl(d, 0, 64);
   const cplx<double> a = gram_entry_w<W>(Gp, c, e), b = gram_entry_w<W>(Gp, e, c);
   const cplx<float> v = cast<float>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
-  dnum = fmax(dnum, (double)cabs(v - old));
-  dden = fmax(dden, (double)cabs(v + old));
+  // squared magnitudes; the square roots are taken once per block
+  const cplx<double> vd = cast<double>(v), od = cast<double>(old);
+  dnum2 = fmax(dnum2, abs2(vd - od));
+  dden2 = fmax(dden2, abs2(vd + od));
   mout[(size_t)pos * 64 + l] = v;
 }
 
@@ -422,6 +425,8 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
       dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
       dden = fmax(dden, __shfl_xor(dden, o, 64));
     }
+    dnum = sqrt(dnum);  // the partial maxima were of squared magnitudes
+    dden = sqrt(dden);
     if (l == 0) {
       atomic_max_nonneg(&st->num_bits, dnum);
       atomic_max_nonneg(&st->den_bits, dden);
