@@ -315,21 +315,28 @@ __global__ __launch_bounds__(64) void k_canon_h(const cplx<double>* __restrict__
       kc.im[r] = 0.0;
     }
     if (active) {
+      // t outermost with the NH row accumulators inside: each sum keeps its
+      // order while the workspace loads stream through a small register window
       const double* ub = Ub + j * 2 * N;
+      double ar[NH], ai[NH];
+#pragma unroll
+      for (int r = 0; r < NH; ++r) ar[r] = ai[r] = 0.0;
+#pragma unroll 2
+      for (int t = 0; t < N; ++t) {
+        const double br = ub[2 * t], bi = ub[2 * t + 1];
+#pragma unroll
+        for (int r = 0; r < NH; ++r) {
+          const double fr = Uf[(r0 + r) * 2 * N + 2 * t], fi = Uf[(r0 + r) * 2 * N + 2 * t + 1];
+          ar[r] = fma(fr, br, fma(-fi, bi, ar[r]));
+          ai[r] = fma(fr, bi, fma(fi, br, ai[r]));
+        }
+      }
       const double sj = sqb[j];
 #pragma unroll
       for (int r = 0; r < NH; ++r) {
-        const double* uf = Uf + (r0 + r) * 2 * N;
-        double ar = 0.0, ai = 0.0;
-#pragma unroll
-        for (int t = 0; t < N; ++t) {
-          const double fr = uf[2 * t], fi = uf[2 * t + 1], br = ub[2 * t], bi = ub[2 * t + 1];
-          ar = fma(fr, br, fma(-fi, bi, ar));
-          ai = fma(fr, bi, fma(fi, br, ai));
-        }
         const double wgt = sqf[r0 + r] * sj;
-        kc.re[r] = wgt * ar;
-        kc.im[r] = -wgt * ai;
+        kc.re[r] = wgt * ar[r];
+        kc.im[r] = -wgt * ai[r];
       }
     }
   }

@@ -20,10 +20,14 @@ bool fast_bp_launch(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R
   return launch_small<kModeBP, R, R>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
 }
 
+// The fused absorb/gates/gauge step at a fixed bond dimension: one-wave
+// kernels (kernels_small.hpp); the FP32 chi = 8, degree-3 case uses MFMA.
 template <typename R>
-bool fast_local_launch(ClassView, int, int, const cplx<R>*, cplx<R>*, const cplx<double>*, int, int, double, int,
-                       double, double, const double*, int, cplx<R>*, hipStream_t) {
-  return false;
+bool fast_local_launch(ClassView cv, int chi_in, int chi_out, const cplx<R>* Tin, cplx<R>* Tout,
+                       const cplx<double>* canon, int rows, int cols, double, int flags, double z_time, double x_time,
+                       const double* lmbd, int lstride, cplx<R>* msgs_init, hipStream_t s) {
+  if (chi_in != chi_out || flags != (kAbsorb | kGates | kGauge)) return false;
+  return launch_small_local<R>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init, s);
 }
 
 template <typename R>
@@ -109,9 +113,10 @@ inline bool fast_local_launch<float>(ClassView cv, int chi_in, int chi_out, cons
                                      cplx<float>* Tout, const cplx<double>* canon, int rows, int cols,
                                      double zz_time, int flags, double z_time, double x_time, const double* lmbd,
                                      int lstride, cplx<float>* msgs_init, hipStream_t s) {
-  if (chi_in != 8 || chi_out != 8 || cv.d != 3 || cv.n == 0 || flags != (kAbsorb | kGates | kGauge) ||
-      cv.zz == nullptr)
-    return false;
+  if (chi_in != chi_out || flags != (kAbsorb | kGates | kGauge) || cv.zz == nullptr) return false;
+  if (chi_in != 8 || cv.d != 3 || cv.n == 0)
+    return launch_small_local<float>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride,
+                                     msgs_init, s);
   hipLaunchKernelGGL((k_mfma8_local64<3, 4, float>), dim3(mfma8_grid(cv.n)), dim3(256), 0, s, cv, Tin, Tout, canon, rows,
                      cols, zz_time, z_time, x_time, lmbd, lstride, msgs_init);
   return true;

@@ -382,4 +382,173 @@ inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, c
   }
 }
 
+// Fused absorb + Z/X gates + symmetric gauge + message re-initialisation at
+// a fixed bond dimension (chi_in = chi_out = CHI), one wave per node, FP64
+// arithmetic on R storage; same operation as k_node_local with all three
+// flags (backends.py:416-432, state.py:142-156, 219-227).  Absorbing the
+// canonicaliser C_k of bond k with the ZZ split is a mode product with the
+// p-dependent matrix W_k^(p)[c][n] = g0 C_k[c][n] + (1 - 2p) g1 C_k[CHI + c][n]
+// (gates from cv.zz), built once per node in LDS.
+template <int D, int CHI>
+struct SmallLocalSmem {
+  cplx<double> X[Sm<D, CHI>::S];
+  cplx<double> Y[Sm<D, CHI>::S];
+  cplx<double> W[D][2][CHI * CHI];  // [bond][p][c * CHI + n]
+  double sl[D][CHI];                // sqrt(lambda) per bond
+  double red[2];                    // cos, sin of the Z-gate angle
+};
+
+template <int D, int CHI, int K>
+__device__ __forceinline__ void smp_absorb(const cplx<double>* X, cplx<double>* Y, const cplx<double> (&W)[2][CHI * CHI]) {
+  constexpr int S = Sm<D, CHI>::S, A = Sm<D, CHI>::A, st = cpow(CHI, D - 1 - K);
+  const int l = threadIdx.x & 63;
+#pragma unroll 4
+  for (int q = 0; q < (S + 63) / 64; ++q) {
+    const int y = l + 64 * q;
+    if (S % 64 == 0 || y < S) {
+      const int nk = (y / st) % CHI;
+      const int base = y - nk * st;
+      const cplx<double>* w = W[y >= A ? 1 : 0];
+      cplx<double> acc = mk(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < CHI; ++c) cfma(acc, X[base + c * st], w[c * CHI + nk]);
+      Y[y] = acc;
+    }
+  }
+}
+
+template <int D, int CHI, typename R>
+__global__ __launch_bounds__(64) void k_small_local(ClassView cv, const cplx<R>* __restrict__ Tin, cplx<R>* Tout,
+                                                    const cplx<double>* __restrict__ canon, int canon_rows,
+                                                    int canon_cols, double z_time, double x_time,
+                                                    const double* __restrict__ lmbd, int lstride,
+                                                    cplx<R>* msgs_init) {
+  using M = Sm<D, CHI>;
+  __shared__ __align__(16) SmallLocalSmem<D, CHI> sm;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63;
+  const double cx = cos(x_time), sx = sin(x_time);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<R>* Tg = Tin + (size_t)i * M::S;
+#pragma unroll 4
+    for (int q = 0; q < (M::S + 63) / 64; ++q) {
+      const int x = l + 64 * q;
+      if (M::S % 64 == 0 || x < M::S) sm.X[x] = cast<double>(Tg[x]);
+    }
+    for (int q = l; q < D * M::Q; q += 64) {
+      const int k = q / M::Q, o = q % M::Q, c = o / CHI, nn = o % CHI;
+      const cplx<double>* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
+      const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
+      const cplx<double> u = zg[0] * C[c * canon_cols + nn], v = zg[1] * C[(CHI + c) * canon_cols + nn];
+      sm.W[k][0][o] = u + v;
+      sm.W[k][1][o] = u - v;
+    }
+    for (int q = l; q < D * CHI; q += 64) {
+      const int k = q / CHI, e = q % CHI;
+      sm.sl[k][e] = sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + e]);
+    }
+    if (l == 0) {
+      const double phi = cv.h[i] * z_time;
+      sm.red[0] = cos(phi);
+      sm.red[1] = sin(phi);
+    }
+    __syncthreads();
+    smp_absorb<D, CHI, 0>(sm.X, sm.Y, sm.W[0]);
+    __syncthreads();
+    smp_absorb<D, CHI, 1>(sm.Y, sm.X, sm.W[1]);
+    __syncthreads();
+    cplx<double>* F = sm.X;
+    if constexpr (D >= 3) {
+      smp_absorb<D, CHI, 2>(sm.X, sm.Y, sm.W[2]);
+      __syncthreads();
+      F = sm.Y;
+    }
+    if constexpr (D == 4) {
+      smp_absorb<D, CHI, 3>(sm.Y, sm.X, sm.W[3]);
+      __syncthreads();
+      F = sm.X;
+    }
+    // Z then X on the physical index, then sqrt(lambda) on every bond and the L2 norm
+    const double cz = sm.red[0], sz = sm.red[1];
+    const cplx<double> f0 = mk(cz, -sz), f1 = mk(cz, sz);
+    double ss = 0.0;
+    for (int a = l; a < M::A; a += 64) {
+      const cplx<double> t0 = F[a] * f0, t1 = F[M::A + a] * f1;
+      double f = 1.0;
+      int rem = a;
+#pragma unroll
+      for (int k = D - 1; k >= 0; --k) {
+        f *= sm.sl[k][rem % CHI];
+        rem /= CHI;
+      }
+      const cplx<double> y0 = mk(f * (cx * t0.x + sx * t1.y), f * (cx * t0.y - sx * t1.x));
+      const cplx<double> y1 = mk(f * (cx * t1.x + sx * t0.y), f * (cx * t1.y - sx * t0.x));
+      F[a] = y0;
+      F[M::A + a] = y1;
+      ss += abs2(y0) + abs2(y1);
+    }
+    for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+    const double inv = 1.0 / sqrt(ss);
+    __syncthreads();
+    cplx<R>* To = Tout + (size_t)i * M::S;
+#pragma unroll 4
+    for (int q = 0; q < (M::S + 63) / 64; ++q) {
+      const int x = l + 64 * q;
+      if (M::S % 64 == 0 || x < M::S) To[x] = cast<R>(scale(F[x], inv));
+    }
+    for (int q = l; q < D * M::Q; q += 64) {
+      const int k = q / M::Q, o = q % M::Q, a = o / CHI, b = o % CHI;
+      const double* lam = lmbd + (size_t)cv.lmbd_pos[k * n + i] * lstride;
+      double tr = 0.0;
+#pragma unroll
+      for (int e = 0; e < CHI; ++e) tr += lam[e];
+      msgs_init[(size_t)cv.in_pos[k * n + i] * M::Q + o] = mk<R>(a == b ? (R)(lam[a] / tr) : (R)0, (R)0);
+    }
+    __syncthreads();
+  }
+}
+
+template <int D, typename R>
+inline bool launch_small_local_d(int chi, ClassView cv, const cplx<R>* Tin, cplx<R>* Tout, const cplx<double>* canon,
+                                 int rows, int cols, double z_time, double x_time, const double* lmbd, int lstride,
+                                 cplx<R>* msgs, hipStream_t s) {
+#define BQA_SMALL_LOCAL(CH)                                                                                      \
+  case CH:                                                                                                       \
+    hipLaunchKernelGGL((k_small_local<D, CH, R>), dim3(small_grid(cv.n)), dim3(64), 0, s, cv, Tin, Tout, canon,  \
+                       rows, cols, z_time, x_time, lmbd, lstride, msgs);                                         \
+    return true;
+  switch (chi) {
+    BQA_SMALL_LOCAL(2)
+    BQA_SMALL_LOCAL(3)
+    BQA_SMALL_LOCAL(4)
+    default: break;
+  }
+  if constexpr (D <= 3) {
+    switch (chi) {
+      BQA_SMALL_LOCAL(5)
+      BQA_SMALL_LOCAL(6)
+      BQA_SMALL_LOCAL(7)
+      BQA_SMALL_LOCAL(8)
+      default: break;
+    }
+  }
+#undef BQA_SMALL_LOCAL
+  return false;
+}
+
+// The fused simple-update tail at a fixed bond dimension; false when the shape
+// is not served (the caller runs k_node_local).
+template <typename R>
+inline bool launch_small_local(ClassView cv, int chi, const cplx<R>* Tin, cplx<R>* Tout, const cplx<double>* canon,
+                               int rows, int cols, double z_time, double x_time, const double* lmbd, int lstride,
+                               cplx<R>* msgs, hipStream_t s) {
+  if (cv.n == 0 || cv.zz == nullptr || !small_shape(cv.d, chi)) return false;
+  switch (cv.d) {
+    case 2: return launch_small_local_d<2, R>(chi, cv, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs, s);
+    case 3: return launch_small_local_d<3, R>(chi, cv, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs, s);
+    case 4: return launch_small_local_d<4, R>(chi, cv, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs, s);
+    default: return false;
+  }
+}
+
 }  // namespace bqa

@@ -361,3 +361,14 @@ def test_small_kernels_match_generic(name, precision):
     finally:
         for job in jobs:
             job.close()
+    # the fused Trotter step (absorb + gates + gauge at a fixed bond dimension) from the same state
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, precision, g) for g in (False, True)]
+    try:
+        for job in jobs:
+            job.run_layer(layers[k]["xtime"], layers[k]["ztime"], ctx_o.pinv_eps, 60, 1e-9, 0.0)
+        r = [job.density()[0] for job in jobs]
+        print(f"[gpu]   fused step |rho small - generic| {np.abs(r[0] - r[1]).max():.2e}", flush=True)
+        assert np.abs(r[0] - r[1]).max() < (2e-5 if precision == "single" else 1e-10)
+    finally:
+        for job in jobs:
+            job.close()
