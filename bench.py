@@ -1,6 +1,6 @@
 """Benchmark: qubit*Trotter-steps/s of the 100k-qubit random 3-regular QUBO anneal on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 8] [--no-accuracy] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--chi 8] [--no-accuracy] [--no-cpu] [--serve-jobs 48]
 
 Workload: random 3-regular graph on 100,000 qubits (configuration model,
 seed 3), QUBO weights uniform integers in [-10, 10] on edges and diagonal,
@@ -16,7 +16,11 @@ The FP32 path is timed.  Accuracy: the same anneal in FP64 on the GPU
 (untimed) gives the per-qubit <Z> error, rounded-spin disagreements and the
 energy error.  cpu_baseline: the FP64 NumPy oracle on a bounded sample of the
 same workload (3,000-qubit instance from the same generator, six Trotter
-steps once the bond dimension reaches chi, one core).
+steps once the bond dimension reaches chi, one core).  serving: a seeded mix
+of smaller jobs arriving over time through bqa_amd.serve (jobs/h, p50/p99
+time-to-solution).  roofline: the chi_max BP-sweep kernel against the FP32
+matrix peak with the reference contraction schedule's flops, the HBM
+fraction beside it.
 """
 from __future__ import annotations
 
@@ -41,6 +45,7 @@ SEED = 3
 BP_EPS = 1e-6
 BP_ITERS = 75
 PINV_EPS = 1e-6
+SERVE_SEED = 5
 
 
 def make_config(n, chi, steps, precision):
@@ -170,6 +175,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chi4", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the serving sample (0 skips it)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="also run the plain copy kernel on BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -233,6 +239,27 @@ def main():
     if not args.no_chi4 and args.chi != 4:
         c4 = one(4, not args.no_accuracy)
         extra["chi4"] = dict(c4["summary"], accuracy=c4.get("accuracy"))
+    serving = None
+    if args.serve_jobs > 0:
+        from bqa_amd.serve import mixed_requests, serve
+        log(f"[bench] serving: {args.serve_jobs} mixed jobs, Poisson arrivals")
+        reqs = mixed_requests(args.serve_jobs, seed=SERVE_SEED, n_min=100, n_max=5000, rate_per_s=8.0,
+                              degrees=(3,), chis=(4, 8))
+        sv = serve(reqs, max_concurrent=8)
+        keep = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "outcomes"}
+        serving = keep(sv)
+        serving["workload"] = (f"{args.serve_jobs} jobs, Poisson arrivals at 8/s (seed {SERVE_SEED}); random 3-regular, "
+                               "n log-uniform in [100, 5000], chi in {4, 8}, J ~ U[-1,1], h = 0, T = 10, "
+                               "100-300 Trotter steps growing with log n; FP32; at most 8 resident jobs")
+        serving["note"] = ("jobs_per_h and p50/p99 at the offered load above; capacity: the same jobs all queued "
+                           "at t = 0")
+        for r in reqs:
+            r.arrival = 0.0
+        cap = serve(reqs, max_concurrent=8)
+        serving["capacity"] = {k: v for k, v in keep(cap).items() if k in ("jobs", "failed", "makespan_s", "jobs_per_h",
+                                                                        "qubit_steps_per_s", "p50_s", "p99_s")}
+        log(f"[bench] serving: {serving['jobs_per_h']:.4g} jobs/h offered, p50 {serving['p50_s']:.3f} s, "
+            f"p99 {serving['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h; failed {serving['failed']}")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (FP64 NumPy oracle)")
@@ -262,6 +289,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": head.get("accuracy"),
+        "serving": serving,
     }
     result.update(extra)
     if copy_ms is not None:
