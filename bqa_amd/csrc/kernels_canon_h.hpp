@@ -87,8 +87,14 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
   for (int o = 1; o < P2; o <<= 1) sc = fmax(sc, __shfl_xor(sc, o, 64));
   const double small2 = 1e-26 * sc;
   const double tol = 2.2204460492503131e-16 * N;
+  // A sweep whose rotations all had column cosine |g| / sqrt(a b) < 1e-8 and
+  // sine |s| < 1e-6 leaves every cosine below ~1e-14 (each later rotation of
+  // the sweep changes a settled pair by at most the product of the two), so
+  // the sweep that would only confirm convergence is skipped; near-degenerate
+  // pairs, where tiny cosines give large rotations, keep the full test.
+  constexpr double kSettledCos2 = 1e-16, kSettledSin2 = 1e-12;
   for (int sweep = 0; sweep < 30; ++sweep) {
-    int rotated = 0;
+    int rotated = 0, unsettled = 0;
 #pragma unroll 1
     for (int rnd = 0; rnd < N - 1; ++rnd) {
       double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
@@ -117,6 +123,7 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
         const double c = rsqrt(fma(t, t, 1.0));
         const double s = c * t;
         const double er = gr * rga, ei = -gi * rga;
+        unsettled |= (g2 > kSettledCos2 * a * b || s * s > kSettledSin2) ? 1 : 0;
 #pragma unroll
         for (int r = 0; r < NH; ++r) {
           const double xr = er * bot.re[r] - ei * bot.im[r];
@@ -130,7 +137,7 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
       }
       hrotate_owners<N>(top, bot, k);
     }
-    if (!__any(rotated)) break;
+    if (!__any(rotated) || !__any(unsettled)) break;
   }
 }
 
