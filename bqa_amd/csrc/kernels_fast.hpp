@@ -46,10 +46,18 @@ bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msg
   if (cv.n == 0 || cv.zz == nullptr) return false;
   if (chi == 8 && (cv.d == 2 || cv.d == 3)) {
     const int grid = std::min(cv.n, 4096);
-    if (cv.d == 3)
-      hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext);
-    else
-      hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext);
+    if constexpr (sizeof(R) == sizeof(float)) {
+      // FP32 job: FP32 MFMA mode products, FP64 MFMA Gram
+      if (cv.d == 3)
+        hipLaunchKernelGGL((k_mfma8_ext32<3, 4>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext);
+      else
+        hipLaunchKernelGGL((k_mfma8_ext32<2, 1>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext);
+    } else {
+      if (cv.d == 3)
+        hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext);
+      else
+        hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext);
+    }
     return true;
   }
   cplx<R>* m = const_cast<cplx<R>*>(msgs);

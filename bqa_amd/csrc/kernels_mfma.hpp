@@ -699,6 +699,109 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx
 namespace bqa {
 
 // ---------------------------------------------------------------------------
+// Extended messages of an FP32 job, chi = 8: FP32 MFMA mode products on the
+// swizzled float layout of k_mfma8, and the p-resolved Gram on FP64 MFMA with
+// the FP32 operands widened on load, so every product and every partial sum of
+// the Gram is FP64.  (The Gram's small eigenvalues come from cancellation over
+// the fibres: with FP32 partial sums the anneal's <Z> error grows by three
+// orders of magnitude; FP32 rounding of the mode products does not matter.)
+// One block of W waves per node; with W >= 2 each wave's Gram k-steps lie in
+// one p half.  The partials go into the dead Y buffer (degree 3) or G.
+template <int D, int W>
+struct Mf8ExtSmem {
+  float T[2 * Mf8<D>::S];
+  float W1[2 * Mf8<D>::S];
+  float W2[2 * Mf8<D>::S];
+  double G[D == 2 ? 512 : 1];
+};
+
+// This wave's k-steps of the p-resolved Gram along bond J from swizzled float
+// T and Y, accumulated on FP64 MFMA (two chains per p for issue overlap).
+template <int D, int W, int J>
+__device__ __forceinline__ void gram8x(const float* T, const float* Y, d4v& g0, d4v& g1, const Swz8Lanes<D>& z) {
+  constexpr int KS = Mf8<D>::KS, KPW = Mf8W<D, W>::KPW;
+  constexpr int B = KPW < 8 ? KPW : 8;
+  const int gl = opaque(z.gl[J]);
+  d4v c[2][2] = {{d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}},
+                 {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}}};
+#pragma unroll
+  for (int s0 = 0; s0 < KPW; s0 += B) {
+    double ta[B], ya[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int off = gl ^ (4 * swz<D>(2 * fbase8<D, J>(4 * (s0 + b))));
+      ta[b] = (double)lds_ld(T, off);
+      ya[b] = (double)lds_ld(Y, off);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int p = W == 1 ? (s0 + b) / (KS / 2) : 0;
+      c[p][b & 1] = mfma4d(ta[b], ya[b], c[p][b & 1]);
+    }
+  }
+  g0 = c[0][0] + c[0][1];
+  g1 = c[1][0] + c[1][1];
+}
+
+template <int D, int W, int J>
+__device__ __forceinline__ void ext_out8x(const float* T, const float* Y, double* Gp, const ClassView& cv, int i,
+                                          cplx<double>* ext_out, const Swz8Lanes<D>& z) {
+  d4v g0, g1;
+  gram8x<D, W, J>(T, Y, g0, g1, z);
+  __syncthreads();
+  store_gd<W>(Gp, g0, g1);
+  __syncthreads();
+  const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[J * cv.n + i];
+  finish8_ext64<W>(Gp, cv.out_pos[J * cv.n + i], ext_out, zg[0], zg[1]);
+  __syncthreads();
+}
+
+template <int D, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma8_ext32(ClassView cv, const cplx<float>* __restrict__ T,
+                                                        const cplx<float>* __restrict__ msgs, cplx<double>* ext_out) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(W == 1 || W % 2 == 0, "single-p Gram slices per wave");
+  static_assert(D == 2 || W * 256 <= Mf8<D>::S, "Gram partials fit in a dead work buffer");
+  __shared__ __align__(16) Mf8ExtSmem<D, W> sm;
+  const int n = cv.n;
+  constexpr int S = Mf8<D>::S;
+  const Swz8Lanes<D> zl = swz8_lanes<D, W>();
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    load_tensor8<D, W>(T + (size_t)i * S, sm.T);
+    float bm[D][4];
+#pragma unroll
+    for (int j = 0; j < D; ++j) msg_operand8(msgs + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
+    __syncthreads();
+    if constexpr (D == 2) {
+      mp8<D, W, 1>(sm.T, sm.W1, bm[1], zl);          // Y0 = T x1 m1
+      __syncthreads();
+      ext_out8x<D, W, 0>(sm.T, sm.W1, sm.G, cv, i, ext_out, zl);
+      mp8<D, W, 0>(sm.T, sm.W2, bm[0], zl);          // Y1 = T x0 m0
+      __syncthreads();
+      ext_out8x<D, W, 1>(sm.T, sm.W2, sm.G, cv, i, ext_out, zl);
+    } else {
+      mp8<D, W, 2>(sm.T, sm.W1, bm[2], zl);          // W1 = T x2 m2
+      __syncthreads();
+      mp8<D, W, 1>(sm.W1, sm.W2, bm[1], zl);         // Y0 = W1 x1 m1
+      __syncthreads();
+      ext_out8x<D, W, 0>(sm.T, sm.W2, reinterpret_cast<double*>(sm.W2), cv, i, ext_out, zl);
+      mp8<D, W, 0>(sm.W1, sm.W2, bm[0], zl);         // Y1 = W1 x0 m0
+      __syncthreads();
+      ext_out8x<D, W, 1>(sm.T, sm.W2, reinterpret_cast<double*>(sm.W2), cv, i, ext_out, zl);
+      mp8<D, W, 0>(sm.T, sm.W1, bm[0], zl);          // W1 = T x0 m0
+      __syncthreads();
+      mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);         // Y2 = W1 x1 m1 (in place)
+      __syncthreads();
+      ext_out8x<D, W, 2>(sm.T, sm.W1, reinterpret_cast<double*>(sm.W1), cv, i, ext_out, zl);
+    }
+  }
+}
+
+}  // namespace bqa
+
+namespace bqa {
+
+// ---------------------------------------------------------------------------
 // Fused absorb + Z/X gates + symmetric gauge + message re-initialisation,
 // chi_in = chi_out = 8, degree 3 (src/bqa/backends.py:416-432,
 // state.py:142-156, 219-227), tensors stored in R (FP32 or FP64) and all
