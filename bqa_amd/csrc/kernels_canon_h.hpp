@@ -3,9 +3,15 @@
 // Same results as k_canonicalize / k_canon_reg, organised for occupancy and
 // fewer Jacobi sweeps:
 //   * 2 * P2 lanes per edge (P2 = n/2 rounded up to a power of two): lane
-//     (k, h) holds rows [h n/2, (h+1) n/2) of the two columns of pair k, so a
-//     column dot product is a lane-local sum plus one exchange with the other
-//     half (lane ^ P2); columns move between pairs with DPP row shifts;
+//     (k, h) = k + P2 h holds rows [h n/2, (h+1) n/2) of the two columns of
+//     pair k, so a column dot product is a lane-local sum plus one DPP
+//     exchange with the other half (lane ^ P2);
+//   * when n/2 is a power of two the Jacobi sweep pairs columns in a recursive
+//     xor order (all top x bot pairs with only the bot columns moving by one
+//     lane-xor per round, then each half split in two by one re-layout, and
+//     so on: n - 1 rounds), so a round moves one column per lane, through the
+//     LDS crossbar (ds_bpermute) rather than VALU moves and selects; other n
+//     use a round-robin with DPP row shifts;
 //   * the Hermitian PSD extended messages are factored M = L L^H by pivoted
 //     Cholesky first; one-sided Jacobi on the columns of L then converges in
 //     about half the sweeps of Jacobi on M and yields V_M Sigma_L directly,
@@ -48,9 +54,33 @@ __device__ __forceinline__ double dpp_from_prev(double x) {  // value of lane - 
   return __hiloint2double(hi, lo);
 }
 
+// 32-bit DPP move within 16-lane rows (every source lane valid for the
+// patterns used here).
+template <int CTRL>
+__device__ __forceinline__ int dpp32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
+
+// Value of lane ^ M (M = 1, 2, 4, 8) within 16-lane rows: quad permutations
+// for 1 and 2, half-row mirror (lane ^ 7) then lane ^ 3 for 4, rotation by 8.
+template <int M>
+__device__ __forceinline__ int xor_lane32(int v) {
+  static_assert(M == 1 || M == 2 || M == 4 || M == 8, "lane xor within a row");
+  if constexpr (M == 1) return dpp32<0xB1>(v);
+  else if constexpr (M == 2) return dpp32<0x4E>(v);
+  else if constexpr (M == 4) return dpp32<0x1B>(dpp32<0x141>(v));
+  else return dpp32<0x128>(v);
+}
+
+template <int M>
+__device__ __forceinline__ double xor_lane(double x) {
+  const int lo = xor_lane32<M>(__double2loint(x)), hi = xor_lane32<M>(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
 template <int N>
 __device__ __forceinline__ double half_sum(double v) {
-  return v + __shfl_xor(v, HCanon<N>::P2, 64);
+  return v + xor_lane<HCanon<N>::P2>(v);
 }
 
 template <int N>
@@ -78,101 +108,151 @@ __device__ __forceinline__ double hnorm2(const HCol<NH>& c) {
   return s;
 }
 
-// One-sided Jacobi on the pair of split columns held by this lane pair.
+// One Jacobi round: rotate the pair of split columns held by this lane pair
+// (top, bot) so that they become orthogonal; `rotated` / `unsettled` collect
+// whether the round rotated at all and whether a rotation was still large.
 template <int N>
-__device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) {
-  constexpr int NH = N / 2, P2 = HCanon<N>::P2;
-  double sc = active ? fmax(half_sum<N>(hnorm2(top)), half_sum<N>(hnorm2(bot))) : 0.0;
-#pragma unroll
-  for (int o = 1; o < P2; o <<= 1) sc = fmax(sc, __shfl_xor(sc, o, 64));
-  const double small2 = 1e-26 * sc;
-  const double tol = 2.2204460492503131e-16 * N;
+__device__ __forceinline__ void hjacobi_round(HCol<N / 2>& top, HCol<N / 2>& bot, bool active, double small2,
+                                              double tol, int& rotated, int& unsettled) {
+  constexpr int NH = N / 2;
   // A sweep whose rotations all had column cosine |g| / sqrt(a b) < 1e-8 and
   // sine |s| < 1e-6 leaves every cosine below ~1e-14 (each later rotation of
   // the sweep changes a settled pair by at most the product of the two), so
   // the sweep that would only confirm convergence is skipped; near-degenerate
   // pairs, where tiny cosines give large rotations, keep the full test.
   constexpr double kSettledCos2 = 1e-16, kSettledSin2 = 1e-12;
+  double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+  for (int r = 0; r < NH; ++r) {
+    a = fma(top.re[r], top.re[r], fma(top.im[r], top.im[r], a));
+    b = fma(bot.re[r], bot.re[r], fma(bot.im[r], bot.im[r], b));
+    gr = fma(top.re[r], bot.re[r], fma(top.im[r], bot.im[r], gr));
+    gi = fma(top.re[r], bot.im[r], fma(-top.im[r], bot.re[r], gi));
+  }
+  a = half_sum<N>(a);
+  b = half_sum<N>(b);
+  gr = half_sum<N>(gr);
+  gi = half_sum<N>(gi);
+  const double g2 = gr * gr + gi * gi;
+  if (active && fmin(a, b) >= small2 && g2 > tol * tol * a * b) {
+    rotated = 1;
+    // rotation parameters from reciprocal square roots (c^2 + s^2 = 1 to a few ulp)
+    const double rga = rsqrt(g2);
+    const double zeta = 0.5 * (b - a) * rga;
+    const double w = fabs(zeta) + sqrt(fma(zeta, zeta, 1.0));
+    double rw = __builtin_amdgcn_rcp(w);
+    rw = fma(fma(-w, rw, 1.0), rw, rw);
+    rw = fma(fma(-w, rw, 1.0), rw, rw);
+    const double t = zeta >= 0.0 ? rw : -rw;
+    const double c = rsqrt(fma(t, t, 1.0));
+    const double s = c * t;
+    const double er = gr * rga, ei = -gi * rga;
+    unsettled |= (g2 > kSettledCos2 * a * b || s * s > kSettledSin2) ? 1 : 0;
+#pragma unroll
+    for (int r = 0; r < NH; ++r) {
+      const double xr = er * bot.re[r] - ei * bot.im[r];
+      const double xi = er * bot.im[r] + ei * bot.re[r];
+      const double pr = top.re[r], pi = top.im[r];
+      bot.re[r] = s * pr + c * xr;
+      bot.im[r] = s * pi + c * xi;
+      top.re[r] = c * pr - s * xr;
+      top.im[r] = c * pi - s * xi;
+    }
+  }
+}
+
+// Value of lane `src` (a wave lane, passed as 4 * src) through the LDS
+// crossbar: no VALU work beyond the address, for the column moves.
+__device__ __forceinline__ double bperm(double x, int addr4) {
+  const int lo = __builtin_amdgcn_ds_bpermute(addr4, __double2loint(x));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr4, __double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
+// One-sided Jacobi on the columns held by the lane pairs of an edge.
+template <int N>
+__device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) {
+  constexpr int NH = N / 2, P = N / 2, P2 = HCanon<N>::P2;
+  double sc = active ? fmax(half_sum<N>(hnorm2(top)), half_sum<N>(hnorm2(bot))) : 0.0;
+#pragma unroll
+  for (int o = 1; o < P2; o <<= 1) sc = fmax(sc, __shfl_xor(sc, o, 64));
+  const double small2 = 1e-26 * sc;
+  const double tol = 2.2204460492503131e-16 * N;
   for (int sweep = 0; sweep < 30; ++sweep) {
     int rotated = 0, unsettled = 0;
+    if constexpr (P == P2) {
+      // recursive xor order: in the phase of group size L the tops and the
+      // bots of each group of L lanes meet all pairs (bot at lane k holds the
+      // bot that started at lane k ^ gray(t) in round t, so round t moves it by
+      // the lowest set bit of t); the re-layout at bit L / 2 then leaves the
+      // pairs within each half for size L / 2: lanes with the bit clear take
+      // (own top, partner's top), lanes with it set (own bot, partner's bot)
+      const int lane = threadIdx.x & 63;
 #pragma unroll 1
-    for (int rnd = 0; rnd < N - 1; ++rnd) {
-      double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
+      for (int L = P; L >= 1; L >>= 1) {
+#pragma unroll 1
+        for (int t = 0; t < L; ++t) {
+          if (t > 0) {
+            const int addr = 4 * (lane ^ (t & -t));
 #pragma unroll
-      for (int r = 0; r < NH; ++r) {
-        a = fma(top.re[r], top.re[r], fma(top.im[r], top.im[r], a));
-        b = fma(bot.re[r], bot.re[r], fma(bot.im[r], bot.im[r], b));
-        gr = fma(top.re[r], bot.re[r], fma(top.im[r], bot.im[r], gr));
-        gi = fma(top.re[r], bot.im[r], fma(-top.im[r], bot.re[r], gi));
-      }
-      a = half_sum<N>(a);
-      b = half_sum<N>(b);
-      gr = half_sum<N>(gr);
-      gi = half_sum<N>(gi);
-      const double g2 = gr * gr + gi * gi;
-      if (active && fmin(a, b) >= small2 && g2 > tol * tol * a * b) {
-        rotated = 1;
-        // rotation parameters from reciprocal square roots (c^2 + s^2 = 1 to a few ulp)
-        const double rga = rsqrt(g2);
-        const double zeta = 0.5 * (b - a) * rga;
-        const double w = fabs(zeta) + sqrt(fma(zeta, zeta, 1.0));
-        double rw = __builtin_amdgcn_rcp(w);
-        rw = fma(fma(-w, rw, 1.0), rw, rw);
-        rw = fma(fma(-w, rw, 1.0), rw, rw);
-        const double t = zeta >= 0.0 ? rw : -rw;
-        const double c = rsqrt(fma(t, t, 1.0));
-        const double s = c * t;
-        const double er = gr * rga, ei = -gi * rga;
-        unsettled |= (g2 > kSettledCos2 * a * b || s * s > kSettledSin2) ? 1 : 0;
+            for (int r = 0; r < NH; ++r) {
+              bot.re[r] = bperm(bot.re[r], addr);
+              bot.im[r] = bperm(bot.im[r], addr);
+            }
+          }
+          hjacobi_round<N>(top, bot, active, small2, tol, rotated, unsettled);
+        }
+        if (L > 1) {
+          const int m = L >> 1, addr = 4 * (lane ^ m);
+          const bool hi = (k & m) != 0;
 #pragma unroll
-        for (int r = 0; r < NH; ++r) {
-          const double xr = er * bot.re[r] - ei * bot.im[r];
-          const double xi = er * bot.im[r] + ei * bot.re[r];
-          const double pr = top.re[r], pi = top.im[r];
-          top.re[r] = c * pr - s * xr;
-          top.im[r] = c * pi - s * xi;
-          bot.re[r] = s * pr + c * xr;
-          bot.im[r] = s * pi + c * xi;
+          for (int r = 0; r < NH; ++r) {
+            const double fr = bperm(hi ? top.re[r] : bot.re[r], addr);
+            const double fi = bperm(hi ? top.im[r] : bot.im[r], addr);
+            top.re[r] = hi ? bot.re[r] : top.re[r];
+            top.im[r] = hi ? bot.im[r] : top.im[r];
+            bot.re[r] = fr;
+            bot.im[r] = fi;
+          }
         }
       }
-      hrotate_owners<N>(top, bot, k);
+    } else {
+#pragma unroll 1
+      for (int rnd = 0; rnd < N - 1; ++rnd) {
+        hjacobi_round<N>(top, bot, active, small2, tol, rotated, unsettled);
+        hrotate_owners<N>(top, bot, k);
+      }
     }
     if (!__any(rotated) || !__any(unsettled)) break;
   }
 }
 
-// Element `idx` (0..N-1) of a split column: the half holding it contributes, the other adds 0.
-template <int N>
-__device__ __forceinline__ void helem(const HCol<N / 2>& c, int idx, int h, double& re, double& im) {
-  constexpr int NH = N / 2;
-  double xr = 0.0, xi = 0.0;
-  const int loc = idx - h * NH;
-#pragma unroll
-  for (int r = 0; r < NH; ++r) {
-    xr = r == loc ? c.re[r] : xr;
-    xi = r == loc ? c.im[r] : xi;
-  }
-  re = half_sum<N>(xr);
-  im = half_sum<N>(xi);
+// Orders this wave's LDS accesses around a cross-lane exchange (the block is
+// one wave, whose LDS operations complete in issue order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Pivoted Cholesky M = L L^H in place: each column ends up holding one column of
-// L (in pivot order) or zero beyond the numerical rank.
+// L (in pivot order) or zero beyond the numerical rank.  The scaled pivot
+// column goes through the edge's LDS slot `lcol` (N complex).  dgt / dgb are the
+// residual diagonals of the lane's columns ct = k and cb = N - 1 - k, updated
+// with the arithmetic the column update applies to its diagonal element.
 template <int N>
-__device__ void hcholesky(HCol<N / 2>& top, HCol<N / 2>& bot, int k, int h, int base, bool active) {
+__device__ void hcholesky(HCol<N / 2>& top, HCol<N / 2>& bot, int k, int h, bool active, double dgt, double dgb,
+                          double2* lcol) {
   constexpr int NH = N / 2, P2 = HCanon<N>::P2;
-  const int ct = k, cb = N - 1 - k;  // matrix columns held (fixed during the factorisation)
-  bool dt = !active, db = !active;   // done flags
+  const int ct = k, cb = N - 1 - k, r0 = h * NH;
+  bool dt = !active, db = !active;  // done flags
   double d0 = 0.0;
   for (int step = 0; step < N; ++step) {
-    double tr, ti, br, bi;
-    helem<N>(top, ct, h, tr, ti);
-    helem<N>(bot, cb, h, br, bi);
     // best undone diagonal over the group: value and the matrix column holding it
     double bv = -1.0;
     int bc = -1;
-    if (!dt && tr > bv) { bv = tr; bc = ct; }
-    if (!db && br > bv) { bv = br; bc = cb; }
+    if (!dt && dgt > bv) { bv = dgt; bc = ct; }
+    if (!db && dgb > bv) { bv = dgb; bc = cb; }
 #pragma unroll
     for (int o = 1; o < P2; o <<= 1) {
       const double ov = __shfl_xor(bv, o, 64);
@@ -191,53 +271,53 @@ __device__ void hcholesky(HCol<N / 2>& top, HCol<N / 2>& bot, int k, int h, int 
       }
       break;
     }
-    // pivot column, scaled: l = R[:, bc] / sqrt(R[bc][bc]); owner lane pair of column bc
+    // pivot column, scaled: l = R[:, bc] / sqrt(R[bc][bc]), written by the lane
+    // pair holding column bc
     const int owner_k = bc < N / 2 ? bc : N - 1 - bc;
     const bool own_top = bc == owner_k;
     const double inv = 1.0 / sqrt(bv);
-    const int src = base + owner_k + h * P2;
+    if (k == owner_k) {
+#pragma unroll
+      for (int r = 0; r < NH; ++r)
+        lcol[r0 + r] = make_double2((own_top ? top.re[r] : bot.re[r]) * inv, (own_top ? top.im[r] : bot.im[r]) * inv);
+    }
+    wave_lds_sync();
     HCol<NH> l;
 #pragma unroll
     for (int r = 0; r < NH; ++r) {
-      const double atr = __shfl(top.re[r], src, 64), ati = __shfl(top.im[r], src, 64);
-      const double abr = __shfl(bot.re[r], src, 64), abi = __shfl(bot.im[r], src, 64);
-      l.re[r] = (own_top ? atr : abr) * inv;
-      l.im[r] = (own_top ? ati : abi) * inv;
+      const double2 v = lcol[r0 + r];
+      l.re[r] = v.x;
+      l.im[r] = v.y;
     }
     // conj(l[j]) for this lane's two columns j
-    double ltr, lti, lbr, lbi;
-    helem<N>(l, ct, h, ltr, lti);
-    helem<N>(l, cb, h, lbr, lbi);
+    const double2 lt = lcol[ct], lb = lcol[cb];
+    wave_lds_sync();
+    // one in-place form for every column: R[:, j] <- a R[:, j] - l conj(b l[j]),
+    // with (a, b) = (1, 1) to update, (1 / sqrt(R[bc][bc]), 0) for the pivot
+    // column (which becomes l) and (1, 0) once done
     const bool pt = active && bc == ct, pb = active && bc == cb;
-    if (!dt && !pt) {
+    const double at = pt ? inv : 1.0, ab = pb ? inv : 1.0;
+    const double ut = (!dt && !pt) ? 1.0 : 0.0, ub = (!db && !pb) ? 1.0 : 0.0;
+    const double ltr = ut * lt.x, lti = ut * lt.y, lbr = ub * lb.x, lbi = ub * lb.y;
 #pragma unroll
-      for (int r = 0; r < NH; ++r) {  // R[:, ct] -= l conj(l[ct])
-        top.re[r] -= l.re[r] * ltr + l.im[r] * lti;
-        top.im[r] -= l.im[r] * ltr - l.re[r] * lti;
-      }
+    for (int r = 0; r < NH; ++r) {
+      const double tr = fma(l.re[r], ltr, l.im[r] * lti), ti = fma(l.im[r], ltr, -(l.re[r] * lti));
+      const double br = fma(l.re[r], lbr, l.im[r] * lbi), bi = fma(l.im[r], lbr, -(l.re[r] * lbi));
+      top.re[r] = fma(at, top.re[r], -tr);
+      top.im[r] = fma(at, top.im[r], -ti);
+      bot.re[r] = fma(ab, bot.re[r], -br);
+      bot.im[r] = fma(ab, bot.im[r], -bi);
     }
-    if (!db && !pb) {
-#pragma unroll
-      for (int r = 0; r < NH; ++r) {
-        bot.re[r] -= l.re[r] * lbr + l.im[r] * lbi;
-        bot.im[r] -= l.im[r] * lbr - l.re[r] * lbi;
-      }
-    }
-    if (pt) {
-#pragma unroll
-      for (int r = 0; r < NH; ++r) { top.re[r] = l.re[r]; top.im[r] = l.im[r]; }
-      dt = true;
-    }
-    if (pb) {
-#pragma unroll
-      for (int r = 0; r < NH; ++r) { bot.re[r] = l.re[r]; bot.im[r] = l.im[r]; }
-      db = true;
-    }
+    // the diagonal element's update: (l[j] conj(l[j])) in the form above
+    dgt = fma(1.0, dgt, -fma(ltr, ltr, lti * lti));
+    dgb = fma(1.0, dgb, -fma(lbr, lbr, lbi * lbi));
+    dt = dt || pt;
+    db = db || pb;
   }
 }
 
 template <int N>
-__global__ __launch_bounds__(64) void k_canon_h(const cplx<double>* __restrict__ ext, int E, int ccols,
+__global__ __launch_bounds__(64, 3) void k_canon_h(const cplx<double>* __restrict__ ext, int E, int ccols,
                                                 double pinv_eps, double deps, zd* __restrict__ canon,
                                                 double* __restrict__ lfull, double* __restrict__ ws) {
   using S = HCanon<N>;
@@ -258,9 +338,12 @@ __global__ __launch_bounds__(64) void k_canon_h(const cplx<double>* __restrict__
   double* ipb = sqb + N;
   const int r0 = h * NH;
 
+  __shared__ double2 lslots[S::EPW * N];
+  double2* lcol = lslots + (lane / G) * N;
   HCol<NH> top, bot;
   for (int side = 0; side < 2; ++side) {
     const int ct = k, cb = N - 1 - k;
+    double dgt = 0.0, dgb = 0.0;
 #pragma unroll
     for (int r = 0; r < NH; ++r) {
       top.re[r] = top.im[r] = bot.re[r] = bot.im[r] = 0.0;
@@ -269,18 +352,30 @@ __global__ __launch_bounds__(64) void k_canon_h(const cplx<double>* __restrict__
       // Hermitian part (M + M^H) / 2: the extended message is Hermitian by
       // construction, and its anti-Hermitian rounding part (from FP32 messages)
       // is dropped rather than folded into the factorisation.
+      // (columns and rows from four base pointers with constant offsets, one
+      // column at a time, so few loads and no per-row addresses stay live)
       const cplx<double>* M = ext + (size_t)(side ? E + edge : edge) * N * N;
+      const cplx<double>* Mt = M + r0 * N + ct;  // column ct from row r0
+      const cplx<double>* Rt = M + ct * N + r0;  // row ct from column r0
+      const cplx<double>* Mb = M + r0 * N + cb;
+      const cplx<double>* Rb = M + cb * N + r0;
 #pragma unroll
       for (int r = 0; r < NH; ++r) {
-        const cplx<double> x = M[(r0 + r) * N + ct], y = M[(r0 + r) * N + cb];
-        const cplx<double> xt = M[ct * N + r0 + r], yt = M[cb * N + r0 + r];
+        const cplx<double> x = Mt[r * N], xt = Rt[r];
         top.re[r] = 0.5 * (x.x + xt.x);
         top.im[r] = 0.5 * (x.y - xt.y);
+      }
+      dgt = M[ct * N + ct].x;
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int r = 0; r < NH; ++r) {
+        const cplx<double> y = Mb[r * N], yt = Rb[r];
         bot.re[r] = 0.5 * (y.x + yt.x);
         bot.im[r] = 0.5 * (y.y - yt.y);
       }
+      dgb = M[cb * N + cb].x;
     }
-    hcholesky<N>(top, bot, k, h, base, active);
+    hcholesky<N>(top, bot, k, h, active, dgt, dgb, lcol);
     hjacobi<N>(top, bot, k, active);
     // columns are V_M sigma: eigenvalue s = sigma^2, U = column / sigma
     double* U = side ? Ub : Uf;
