@@ -12,6 +12,29 @@
 
 namespace bqa {
 
+// Blocks of kernel K (THREADS per block) resident on the device at once: CUs x
+// resident blocks per CU, computed once per kernel.  The node kernels loop over
+// their nodes with a grid stride, so one resident round of blocks keeps every
+// CU busy to the end, where a larger grid leaves a partly filled last round.
+template <auto K, int THREADS>
+inline int resident_blocks() {
+  static const int blocks = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      return 4096;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, K, THREADS, 0) != hipSuccess || per <= 0) return 4096;
+    return cus * per;
+  }();
+  return blocks;
+}
+
+template <auto K, int THREADS>
+inline int node_grid(int n) {
+  const int r = resident_blocks<K, THREADS>();
+  return n < r ? n : r;
+}
+
 // Compile-time-shaped one-wave kernels (kernels_small.hpp) for every (d, chi)
 // they serve; the FP32 chi = 8 specialisations below use MFMA instead.
 template <typename R>
@@ -45,18 +68,21 @@ bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msg
                      hipStream_t s) {
   if (cv.n == 0 || cv.zz == nullptr) return false;
   if (chi == 8 && (cv.d == 2 || cv.d == 3)) {
-    const int grid = std::min(cv.n, 4096);
     if constexpr (sizeof(R) == sizeof(float)) {
       // FP32 job: FP32 MFMA mode products, FP64 MFMA Gram
       if (cv.d == 3)
-        hipLaunchKernelGGL((k_mfma8_ext32<3, 4>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext);
+        hipLaunchKernelGGL((k_mfma8_ext32<3, 4>), dim3(node_grid<k_mfma8_ext32<3, 4>, 256>(cv.n)), dim3(256), 0, s,
+                           cv, T, msgs, ext);
       else
-        hipLaunchKernelGGL((k_mfma8_ext32<2, 1>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext);
+        hipLaunchKernelGGL((k_mfma8_ext32<2, 1>), dim3(node_grid<k_mfma8_ext32<2, 1>, 64>(cv.n)), dim3(64), 0, s, cv,
+                           T, msgs, ext);
     } else {
       if (cv.d == 3)
-        hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(grid), dim3(256), 0, s, cv, T, msgs, ext);
+        hipLaunchKernelGGL((k_mfma8_ext64<3, 4, R>), dim3(node_grid<k_mfma8_ext64<3, 4, R>, 256>(cv.n)), dim3(256),
+                           0, s, cv, T, msgs, ext);
       else
-        hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(grid), dim3(64), 0, s, cv, T, msgs, ext);
+        hipLaunchKernelGGL((k_mfma8_ext64<2, 1, R>), dim3(node_grid<k_mfma8_ext64<2, 1, R>, 64>(cv.n)), dim3(64), 0,
+                           s, cv, T, msgs, ext);
     }
     return true;
   }
@@ -65,7 +91,6 @@ bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msg
 }
 
 // FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp).
-inline int mfma8_grid(int n) { return n < 4096 ? n : 4096; }
 // k_mfma8: one block of W waves per node (degree 3: W = 1, 2 or 4; degree 2: W = 1).
 inline int mfma8_waves() {
   static const int w = [] {
@@ -79,15 +104,16 @@ inline int mfma8_waves() {
 template <int MODE, int W>
 inline void launch_mfma8_3(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
                            double* rho, double* bloch, hipStream_t s) {
-  hipLaunchKernelGGL((k_mfma8<3, MODE, W>), dim3(mfma8_grid(cv.n)), dim3(64 * W), 0, s, cv, T, mA, mB, st, rho,
-                     bloch);
+  hipLaunchKernelGGL((k_mfma8<3, MODE, W>), dim3(node_grid<k_mfma8<3, MODE, W>, 64 * W>(cv.n)), dim3(64 * W), 0, s,
+                     cv, T, mA, mB, st, rho, bloch);
 }
 
 template <int MODE>
 inline void launch_mfma8(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
                          double* rho, double* bloch, hipStream_t s) {
   if (cv.d == 2) {
-    hipLaunchKernelGGL((k_mfma8<2, MODE, 1>), dim3(mfma8_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st, rho, bloch);
+    hipLaunchKernelGGL((k_mfma8<2, MODE, 1>), dim3(node_grid<k_mfma8<2, MODE, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
+                       mA, mB, st, rho, bloch);
     return;
   }
   switch (mfma8_waves()) {
@@ -125,8 +151,9 @@ inline bool fast_local_launch<float>(ClassView cv, int chi_in, int chi_out, cons
   if (chi_in != 8 || cv.d != 3 || cv.n == 0)
     return launch_small_local<float>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride,
                                      msgs_init, s);
-  hipLaunchKernelGGL((k_mfma8_local64<3, 4, float>), dim3(mfma8_grid(cv.n)), dim3(256), 0, s, cv, Tin, Tout, canon, rows,
-                     cols, zz_time, z_time, x_time, lmbd, lstride, msgs_init);
+  hipLaunchKernelGGL((k_mfma8_local64<3, 4, float>), dim3(node_grid<k_mfma8_local64<3, 4, float>, 256>(cv.n)),
+                     dim3(256), 0, s, cv, Tin, Tout, canon, rows, cols, zz_time, z_time, x_time, lmbd, lstride,
+                     msgs_init);
   return true;
 }
 

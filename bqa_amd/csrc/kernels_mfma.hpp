@@ -160,6 +160,28 @@ __device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[
 // four interleaved MFMA chains of short FP32 sums, operands read eight k-steps
 // ahead, combined in FP64 into Gp (16 x 16, row-major).  One sequential FP32
 // chain over all fibres measurably degrades the FP32 BP fixed point.
+// Gram operands of k-steps [s0, s0 + B) along bond J from swizzled T and Y.
+template <int D, int J, int B, typename F>
+__device__ __forceinline__ void lds_batch(const F* T, const F* Y, int gl, int s0, F (&ta)[B], F (&ya)[B]) {
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const int off = gl ^ (4 * swz<D>(2 * fbase8<D, J>(4 * (s0 + b))));
+    ta[b] = lds_ld(T, off);
+    ya[b] = lds_ld(Y, off);
+  }
+}
+
+// Orders the LDS reads issued so far ahead of every use of (ta, ya): the empty
+// asm takes the operands as read-write and clobbers memory, so the reads of the
+// next batch stay above it and this batch's MFMAs below it.  Without it the
+// scheduler sinks each read next to its MFMA, and every k-step waits out a full
+// LDS round trip.
+template <int B, typename F>
+__device__ __forceinline__ void pin_after_reads(F (&ta)[B], F (&ya)[B]) {
+#pragma unroll
+  for (int b = 0; b < B; ++b) asm volatile("" : "+v"(ta[b]), "+v"(ya[b]) : : "memory");
+}
+
 template <int D, int W, int J>
 __device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gp, const Swz8Lanes<D>& z) {
   constexpr int KPW = Mf8W<D, W>::KPW;
@@ -169,17 +191,15 @@ __device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gp
   f4v g[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) g[q] = f4v{0.f, 0.f, 0.f, 0.f};
+  float ta[2][B], ya[2][B];
+  lds_batch<D, J, B>(T, Y, gl, 0, ta[0], ya[0]);
 #pragma unroll
   for (int s0 = 0; s0 < KPW; s0 += B) {
-    float ta[B], ya[B];
+    const int c = (s0 / B) & 1;
+    if (s0 + B < KPW) lds_batch<D, J, B>(T, Y, gl, s0 + B, ta[c ^ 1], ya[c ^ 1]);
+    pin_after_reads<B>(ta[c], ya[c]);
 #pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const int off = gl ^ (4 * swz<D>(2 * fbase8<D, J>(4 * (s0 + b))));
-      ta[b] = lds_ld(T, off);
-      ya[b] = lds_ld(Y, off);
-    }
-#pragma unroll
-    for (int b = 0; b < B; ++b) g[b & 3] = mfma4(ta[b], ya[b], g[b & 3]);
+    for (int b = 0; b < B; ++b) g[b & 3] = mfma4(ta[c][b], ya[c][b], g[b & 3]);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -244,7 +264,6 @@ template <int D, int W>
 struct Mf8Smem {
   float T[2 * Mf8<D>::S];
   float W1[2 * Mf8<D>::S];
-  float W2[2 * Mf8<D>::S];
   double Gp[W][256];
 };
 
@@ -286,6 +305,19 @@ __device__ __forceinline__ void msg_operand8(const cplx<float>* __restrict__ m, 
 
 // MODE kModeBP: one BP sweep over the class; kModeDensity: rho and Bloch
 // vectors.  One block of W waves per node.
+// Barrier between the phases of a node.  A one-wave block needs no hardware
+// barrier: its LDS operations complete in issue order, so a compiler fence
+// keeps the accesses in program order while VALU and MFMA work may move across
+// phases (the wave barrier a block barrier becomes would stop that); wider
+// blocks synchronise their waves.
+template <int W>
+__device__ __forceinline__ void node_sync() {
+  if constexpr (W == 1)
+    asm volatile("" ::: "memory");
+  else
+    __syncthreads();
+}
+
 template <int D, int MODE, int W>
 __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
                                                cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
@@ -321,15 +353,15 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
         old[k] = min[(size_t)pos[k] * 64 + l];
       }
     }
-    __syncthreads();
+    node_sync<W>();
     if (MODE == kModeDensity) {
       mp8<D, W, 0>(sm.T, sm.W1, bm[0], zl);
-      __syncthreads();
+      node_sync<W>();
       mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);
-      __syncthreads();
+      node_sync<W>();
       if constexpr (D == 3) {
         mp8<D, W, 2>(sm.W1, sm.W1, bm[2], zl);
-        __syncthreads();
+        node_sync<W>();
       }
       // rho[p][q] = sum_a TM[p, a] conj(T[q, a]), the a range split over the block
       constexpr int A = Mf8<D>::A;
@@ -355,7 +387,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
           sm.Gp[wave][2 * q + 1] = r[q].y;
         }
       }
-      __syncthreads();
+      node_sync<W>();
       if (threadIdx.x == 0) {
         cplx<double> rr[4];
 #pragma unroll
@@ -379,46 +411,51 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
           bo[2] = r00.x - r11.x;
         }
       }
-      __syncthreads();
+      node_sync<W>();
       continue;
     }
     // Each gram8 is followed by a barrier; the first wave then finishes the
     // message while the others start the next mode product, whose own barrier
-    // keeps the partial Grams alive until the first wave has read them.
+    // keeps the partial Grams alive until the first wave has read them.  One
+    // work buffer (mode products run in place along their bond): degree 3
+    // computes T x2 m2 twice, which costs one mode product of 32 MFMAs and
+    // saves 8 KB of LDS per node, so 8 nodes fit per CU instead of 6.
     if constexpr (D == 2) {
       mp8<D, W, 1>(sm.T, sm.W1, bm[1], zl);          // Y0 = T x1 m1
-      __syncthreads();
+      node_sync<W>();
       gram8<D, W, 0>(sm.T, sm.W1, sm.Gp[wave], zl);
-      __syncthreads();
+      node_sync<W>();
       if (lead) finish8<W>(sm.Gp, pos[0], old[0], mout, dnum, dden);
-      mp8<D, W, 0>(sm.T, sm.W2, bm[0], zl);          // Y1 = T x0 m0
-      __syncthreads();
-      gram8<D, W, 1>(sm.T, sm.W2, sm.Gp[wave], zl);
-      __syncthreads();
+      mp8<D, W, 0>(sm.T, sm.W1, bm[0], zl);          // Y1 = T x0 m0
+      node_sync<W>();
+      gram8<D, W, 1>(sm.T, sm.W1, sm.Gp[wave], zl);
+      node_sync<W>();
       if (lead) finish8<W>(sm.Gp, pos[1], old[1], mout, dnum, dden);
     } else {
       mp8<D, W, 2>(sm.T, sm.W1, bm[2], zl);          // W1 = T x2 m2
-      __syncthreads();
-      mp8<D, W, 1>(sm.W1, sm.W2, bm[1], zl);         // Y0 = W1 x1 m1
-      __syncthreads();
-      gram8<D, W, 0>(sm.T, sm.W2, sm.Gp[wave], zl);
-      __syncthreads();
+      node_sync<W>();
+      mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);         // Y0 = W1 x1 m1
+      node_sync<W>();
+      gram8<D, W, 0>(sm.T, sm.W1, sm.Gp[wave], zl);
+      node_sync<W>();
       if (lead) finish8<W>(sm.Gp, pos[0], old[0], mout, dnum, dden);
-      mp8<D, W, 0>(sm.W1, sm.W2, bm[0], zl);         // Y1 = W1 x0 m0
-      __syncthreads();
-      gram8<D, W, 1>(sm.T, sm.W2, sm.Gp[wave], zl);
-      __syncthreads();
+      mp8<D, W, 2>(sm.T, sm.W1, bm[2], zl);          // W1 = T x2 m2
+      node_sync<W>();
+      mp8<D, W, 0>(sm.W1, sm.W1, bm[0], zl);         // Y1 = W1 x0 m0
+      node_sync<W>();
+      gram8<D, W, 1>(sm.T, sm.W1, sm.Gp[wave], zl);
+      node_sync<W>();
       if (lead) finish8<W>(sm.Gp, pos[1], old[1], mout, dnum, dden);
       mp8<D, W, 0>(sm.T, sm.W1, bm[0], zl);          // W1 = T x0 m0
-      __syncthreads();
-      mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);         // Y2 = W1 x1 m1 (in place)
-      __syncthreads();
+      node_sync<W>();
+      mp8<D, W, 1>(sm.W1, sm.W1, bm[1], zl);         // Y2 = W1 x1 m1
+      node_sync<W>();
       gram8<D, W, 2>(sm.T, sm.W1, sm.Gp[wave], zl);
-      __syncthreads();
+      node_sync<W>();
       if (lead) finish8<W>(sm.Gp, pos[2], old[2], mout, dnum, dden);
     }
     // the next node's tensor load overwrites T, which the last Gram read
-    __syncthreads();
+    node_sync<W>();
   }
   if (MODE == kModeBP && lead) {
     for (int o = 32; o > 0; o >>= 1) {
@@ -724,19 +761,17 @@ __device__ __forceinline__ void gram8x(const float* T, const float* Y, d4v& g0, 
   const int gl = opaque(z.gl[J]);
   d4v c[2][2] = {{d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}},
                  {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}}};
+  float ta[2][B], ya[2][B];
+  lds_batch<D, J, B>(T, Y, gl, 0, ta[0], ya[0]);
 #pragma unroll
   for (int s0 = 0; s0 < KPW; s0 += B) {
-    double ta[B], ya[B];
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const int off = gl ^ (4 * swz<D>(2 * fbase8<D, J>(4 * (s0 + b))));
-      ta[b] = (double)lds_ld(T, off);
-      ya[b] = (double)lds_ld(Y, off);
-    }
+    const int q = (s0 / B) & 1;
+    if (s0 + B < KPW) lds_batch<D, J, B>(T, Y, gl, s0 + B, ta[q ^ 1], ya[q ^ 1]);
+    pin_after_reads<B>(ta[q], ya[q]);
 #pragma unroll
     for (int b = 0; b < B; ++b) {
       const int p = W == 1 ? (s0 + b) / (KS / 2) : 0;
-      c[p][b & 1] = mfma4d(ta[b], ya[b], c[p][b & 1]);
+      c[p][b & 1] = mfma4d((double)ta[q][b], (double)ya[q][b], c[p][b & 1]);
     }
   }
   g0 = c[0][0] + c[0][1];
