@@ -156,10 +156,6 @@ __device__ __forceinline__ void mp8(const float* X, float* Y, const float (&bm)[
     for (int v = 0; v < 4; ++v) lds_st(Y, wr ^ (4 * swz<D>(2 * fbase8<D, J>(t * 16 + v))), acc[t][v]);
 }
 
-// This wave's part of G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J:
-// four interleaved MFMA chains of short FP32 sums, operands read eight k-steps
-// ahead, combined in FP64 into Gp (16 x 16, row-major).  One sequential FP32
-// chain over all fibres measurably degrades the FP32 BP fixed point.
 // Gram operands of k-steps [s0, s0 + B) along bond J from swizzled T and Y.
 template <int D, int J, int B, typename F>
 __device__ __forceinline__ void lds_batch(const F* T, const F* Y, int gl, int s0, F (&ta)[B], F (&ya)[B]) {
@@ -182,6 +178,11 @@ __device__ __forceinline__ void pin_after_reads(F (&ta)[B], F (&ya)[B]) {
   for (int b = 0; b < B; ++b) asm volatile("" : "+v"(ta[b]), "+v"(ya[b]) : : "memory");
 }
 
+// This wave's part of G = [Tr | Ti]^T [Yr | Yi] over the fibres along bond J:
+// four interleaved MFMA chains of short FP32 sums, operands read one batch of
+// eight k-steps ahead, combined in FP64 into Gp (16 x 16, row-major).  One
+// sequential FP32 chain over all fibres measurably degrades the FP32 BP fixed
+// point.
 template <int D, int W, int J>
 __device__ __forceinline__ void gram8(const float* T, const float* Y, double* Gp, const Swz8Lanes<D>& z) {
   constexpr int KPW = Mf8W<D, W>::KPW;
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
   __shared__ __align__(16) Mf8Smem<D, W> sm;
   const int n = cv.n;
   const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool lead = wave == 0;
+  const bool lead = W == 1 || wave == 0;  // constant for one wave: no branch around the finish
   cplx<float>* min = msgA;
   cplx<float>* mout = msgB;
   if (MODE == kModeBP) {
