@@ -486,6 +486,130 @@ __global__ __launch_bounds__(64, 3) void k_canon_h(const cplx<double>* __restric
       }
     }
   }
+  if constexpr (N == 16) {
+    if (ccols <= 8) {
+      // chi = 8: the same products regrouped without the 16 x 16 Q,
+      //   V = U_f diag(sq_f) Us and C_f = U_f diag(ip_f) Us in one pass,
+      //   Z[c][j] = keep_b[j] conj((U_b^T V)[j][c]) / Lambda_c,  C_b = U_b Z,
+      // each lane taking a block of 4 rows x 2 columns so every row and column
+      // it loads serves several outputs (about a quarter of the workspace
+      // reads of the entry-per-lane form below).
+      __syncthreads();
+      const int a0 = 4 * (w >> 2), c0 = 2 * (w & 3);
+      double* V = Q;  // V[c][a], 16 x ccols
+      double xr[4][2] = {}, xi[4][2] = {}, yr[4][2] = {}, yi[4][2] = {};
+      if (valid) {
+#pragma unroll 2
+        for (int t = 0; t < N; ++t) {
+          double sr[2], si[2], pr[2], pi[2];
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc) {
+            const int c = c0 + cc;
+            const double ur = c < ccols ? Us[c * 2 * N + 2 * t] : 0.0, ui = c < ccols ? Us[c * 2 * N + 2 * t + 1] : 0.0;
+            sr[cc] = sqf[t] * ur;
+            si[cc] = sqf[t] * ui;
+            pr[cc] = ipf[t] * ur;
+            pi[cc] = ipf[t] * ui;
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double fr = Uf[t * 2 * N + 2 * (a0 + r)], fi = Uf[t * 2 * N + 2 * (a0 + r) + 1];
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              xr[r][cc] = fma(fr, sr[cc], fma(-fi, si[cc], xr[r][cc]));
+              xi[r][cc] = fma(fr, si[cc], fma(fi, sr[cc], xi[r][cc]));
+              yr[r][cc] = fma(fr, pr[cc], fma(-fi, pi[cc], yr[r][cc]));
+              yi[r][cc] = fma(fr, pi[cc], fma(fi, pr[cc], yi[r][cc]));
+            }
+          }
+        }
+        zd* Cf = canon + (size_t)(E + edge) * N * ccols;
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+          const int c = c0 + cc;
+          if (c < ccols) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              V[c * 2 * N + 2 * (a0 + r)] = xr[r][cc];
+              V[c * 2 * N + 2 * (a0 + r) + 1] = xi[r][cc];
+              Cf[(a0 + r) * ccols + c] = mk(yr[r][cc], yi[r][cc]);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // Z rows j = a0..a0+3, columns c0, c0+1
+      double wr[4][2] = {}, wi[4][2] = {};
+      if (valid) {
+#pragma unroll 2
+        for (int u = 0; u < N; ++u) {
+          double vr[2], vi[2];
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc) {
+            const int c = c0 + cc;
+            vr[cc] = c < ccols ? V[c * 2 * N + 2 * u] : 0.0;
+            vi[cc] = c < ccols ? V[c * 2 * N + 2 * u + 1] : 0.0;
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double br = Ub[(a0 + r) * 2 * N + 2 * u], bi = Ub[(a0 + r) * 2 * N + 2 * u + 1];
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              wr[r][cc] = fma(br, vr[cc], fma(-bi, vi[cc], wr[r][cc]));
+              wi[r][cc] = fma(br, vi[cc], fma(bi, vr[cc], wi[r][cc]));
+            }
+          }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+          const int c = c0 + cc;
+          if (c < ccols) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int j = a0 + r;
+              const double g = ipb[j] > 0.0 ? il[c] : 0.0;
+              Zc[(c * N + j) * 2] = wr[r][cc] * g;
+              Zc[(c * N + j) * 2 + 1] = -wi[r][cc] * g;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // C_b rows a0..a0+3, columns c0, c0+1
+      if (valid) {
+        double br_[4][2] = {}, bi_[4][2] = {};
+#pragma unroll 2
+        for (int t = 0; t < N; ++t) {
+          double zr[2], zi[2];
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc) {
+            const int c = c0 + cc;
+            zr[cc] = c < ccols ? Zc[(c * N + t) * 2] : 0.0;
+            zi[cc] = c < ccols ? Zc[(c * N + t) * 2 + 1] : 0.0;
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double vr = Ub[t * 2 * N + 2 * (a0 + r)], vi = Ub[t * 2 * N + 2 * (a0 + r) + 1];
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+              br_[r][cc] = fma(vr, zr[cc], fma(-vi, zi[cc], br_[r][cc]));
+              bi_[r][cc] = fma(vr, zi[cc], fma(vi, zr[cc], bi_[r][cc]));
+            }
+          }
+        }
+        zd* Cb = canon + (size_t)edge * N * ccols;
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+          const int c = c0 + cc;
+          if (c < ccols) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Cb[(a0 + r) * ccols + c] = mk(br_[r][cc], bi_[r][cc]);
+          }
+        }
+      }
+      return;
+    }
+  }
   // Q, one entry per lane per step over the group
   if (valid) {
     for (int x = w; x < N * N; x += G) {
