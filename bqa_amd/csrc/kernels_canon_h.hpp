@@ -111,9 +111,12 @@ __device__ __forceinline__ double hnorm2(const HCol<NH>& c) {
 // One Jacobi round: rotate the pair of split columns held by this lane pair
 // (top, bot) so that they become orthogonal; `rotated` / `unsettled` collect
 // whether the round rotated at all and whether a rotation was still large.
-template <int N>
-__device__ __forceinline__ void hjacobi_round(HCol<N / 2>& top, HCol<N / 2>& bot, bool active, double small2,
-                                              double tol, int& rotated, int& unsettled) {
+// With TRACK the squared column norms a, b are carried (exact at each sweep's
+// start, updated by the rotation's a - t|g|, b + t|g|, moved with their
+// columns), otherwise they are recomputed here.
+template <int N, bool TRACK>
+__device__ __forceinline__ void hjacobi_round(HCol<N / 2>& top, HCol<N / 2>& bot, double& a, double& b, bool active,
+                                              double small2, double tol, int& rotated, int& unsettled) {
   constexpr int NH = N / 2;
   // A sweep whose rotations all had column cosine |g| / sqrt(a b) < 1e-8 and
   // sine |s| < 1e-6 leaves every cosine below ~1e-14 (each later rotation of
@@ -121,16 +124,20 @@ __device__ __forceinline__ void hjacobi_round(HCol<N / 2>& top, HCol<N / 2>& bot
   // the sweep that would only confirm convergence is skipped; near-degenerate
   // pairs, where tiny cosines give large rotations, keep the full test.
   constexpr double kSettledCos2 = 1e-16, kSettledSin2 = 1e-12;
-  double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
+  double gr = 0.0, gi = 0.0;
+  if constexpr (!TRACK) {
+    a = hnorm2(top);
+    b = hnorm2(bot);
+  }
 #pragma unroll
   for (int r = 0; r < NH; ++r) {
-    a = fma(top.re[r], top.re[r], fma(top.im[r], top.im[r], a));
-    b = fma(bot.re[r], bot.re[r], fma(bot.im[r], bot.im[r], b));
     gr = fma(top.re[r], bot.re[r], fma(top.im[r], bot.im[r], gr));
     gi = fma(top.re[r], bot.im[r], fma(-top.im[r], bot.re[r], gi));
   }
-  a = half_sum<N>(a);
-  b = half_sum<N>(b);
+  if constexpr (!TRACK) {
+    a = half_sum<N>(a);
+    b = half_sum<N>(b);
+  }
   gr = half_sum<N>(gr);
   gi = half_sum<N>(gi);
   const double g2 = gr * gr + gi * gi;
@@ -148,6 +155,11 @@ __device__ __forceinline__ void hjacobi_round(HCol<N / 2>& top, HCol<N / 2>& bot
     const double s = c * t;
     const double er = gr * rga, ei = -gi * rga;
     unsettled |= (g2 > kSettledCos2 * a * b || s * s > kSettledSin2) ? 1 : 0;
+    if constexpr (TRACK) {
+      const double tg = t * (g2 * rga);  // t |g|
+      a -= tg;
+      b += tg;
+    }
 #pragma unroll
     for (int r = 0; r < NH; ++r) {
       const double xr = er * bot.re[r] - ei * bot.im[r];
@@ -188,6 +200,7 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
       // pairs within each half for size L / 2: lanes with the bit clear take
       // (own top, partner's top), lanes with it set (own bot, partner's bot)
       const int lane = threadIdx.x & 63;
+      double a = half_sum<N>(hnorm2(top)), b = half_sum<N>(hnorm2(bot));
 #pragma unroll 1
       for (int L = P; L >= 1; L >>= 1) {
 #pragma unroll 1
@@ -199,8 +212,9 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
               bot.re[r] = bperm(bot.re[r], addr);
               bot.im[r] = bperm(bot.im[r], addr);
             }
+            b = bperm(b, addr);
           }
-          hjacobi_round<N>(top, bot, active, small2, tol, rotated, unsettled);
+          hjacobi_round<N, true>(top, bot, a, b, active, small2, tol, rotated, unsettled);
         }
         if (L > 1) {
           const int m = L >> 1, addr = 4 * (lane ^ m);
@@ -214,12 +228,16 @@ __device__ void hjacobi(HCol<N / 2>& top, HCol<N / 2>& bot, int k, bool active) 
             bot.re[r] = fr;
             bot.im[r] = fi;
           }
+          const double fn = bperm(hi ? a : b, addr);
+          a = hi ? b : a;
+          b = fn;
         }
       }
     } else {
 #pragma unroll 1
       for (int rnd = 0; rnd < N - 1; ++rnd) {
-        hjacobi_round<N>(top, bot, active, small2, tol, rotated, unsettled);
+        double a, b;
+        hjacobi_round<N, false>(top, bot, a, b, active, small2, tol, rotated, unsettled);
         hrotate_owners<N>(top, bot, k);
       }
     }
