@@ -154,6 +154,56 @@ def test_phases_against_oracle(name, steps):
         job.close()
 
 
+def _pad_state(st, chi):
+    """The same physical state with every bond zero-padded to dimension chi."""
+    c0 = st.chi
+    tensors = {}
+    for d, t in st.tensors.items():
+        out = np.zeros((t.shape[0], 2) + (chi,) * d, dtype=t.dtype)
+        out[(slice(None), slice(None)) + (slice(0, c0),) * d] = t
+        tensors[d] = out
+    lm = np.zeros((st.lmbds.shape[0], chi), dtype=st.lmbds.dtype)
+    lm[:, :c0] = st.lmbds
+    ms = np.zeros((st.msgs.shape[0], chi, chi), dtype=st.msgs.dtype)
+    ms[:, :c0, :c0] = st.msgs
+    return O.State(tensors=tensors, msgs=ms, lmbds=lm, chi=chi)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_simple_update_truncates_from_chi8(precision):
+    """A chi = 3 mid-anneal state zero-padded to bond dimension 8: the simple update runs the
+    chi = 8 canonicaliser (2 chi = 16) and keeps fewer than 8 columns; chi', the truncation
+    error and the sorted lambdas equal the oracle's."""
+    nodes, edges = instances.regular_uniform(40, 3, seed=24, field_sigma=0.4)
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": 8, "bp_eps": 1e-10, "max_bp_iter_number": 100,
+           "schedule": instances.anneal_schedule(2.0, 24)}
+    ctx_o = O.make_context(cfg)
+    st_o = O.init_state(ctx_o)
+    layers = [i for i in ctx_o.instructions if isinstance(i, dict)]
+    k = 0
+    while st_o.chi < 3:
+        O.run_layer(ctx_o, st_o, layers[k]["xtime"], layers[k]["ztime"])
+        k += 1
+    o2 = _pad_state(st_o, 8)
+    print(f"[gpu] truncation from chi 8 ({precision}): chi {st_o.chi} padded to 8 at step {k}", flush=True)
+    job = Job(config_to_context(dict(cfg, backend="mi355x")), precision)
+    try:
+        job.init_state()
+        _upload(job, ctx_o, o2)
+        O.run_bp(ctx_o, o2)
+        job.run_bp(ctx_o.max_bp_iters, ctx_o.bp_eps, ctx_o.damping)
+        O.simple_update(ctx_o, o2, layers[k]["ztime"])
+        chi, err = job.simple_update(layers[k]["ztime"], ctx_o.pinv_eps)
+        assert 3 <= o2.chi < 8
+        assert chi == o2.chi
+        tol = 1e-10 if precision == "double" else 1e-5
+        assert abs(err - o2.trunc_errs[-1]) < tol
+        lm = np.sort(job.get_lambdas().real, axis=1)
+        assert np.abs(lm - np.sort(o2.lmbds.real, axis=1)).max() < tol
+    finally:
+        job.close()
+
+
 def test_isolated_qubits_and_degree_one():
     """Degree-0 (isolated) and degree-1 classes (the reference asserts on degree 0 in readout)."""
     nodes, edges = instances.heavy_hex_gaussian(2, 3, seed=8)
