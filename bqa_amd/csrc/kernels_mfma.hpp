@@ -851,8 +851,7 @@ namespace bqa {
 // suffice: T -> X -> T -> X.
 template <int D>
 struct Mf8LocalSmemD {
-  double T[2 * Mf8<D>::S];
-  double X[2 * Mf8<D>::S];
+  double T[2 * Mf8<D>::S];  // the mode products run in place
   cplx<double> Wt[D][2][64];
   double sl[D][8];
   double red[4];
@@ -860,7 +859,7 @@ struct Mf8LocalSmemD {
 };
 
 template <int D, int W, typename R>
-__global__ __launch_bounds__(64 * W) void k_mfma8_local64(ClassView cv, const cplx<R>* __restrict__ Tin,
+__global__ __launch_bounds__(64 * W, 5) void k_mfma8_local64(ClassView cv, const cplx<R>* __restrict__ Tin,
                                                           cplx<R>* Tout, const zd* __restrict__ canon,
                                                           int canon_rows, int canon_cols, double zz_time,
                                                           double z_time, double x_time,
@@ -894,21 +893,34 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_local64(ClassView cv, const cp
       sm.zf[1] = sin(phi);
     }
     __syncthreads();
-    double b[D][2][4];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      msg_operand8d(sm.Wt[k][0], b[k][0]);
-      msg_operand8d(sm.Wt[k][1], b[k][1]);
+    // each bond's operands are built from LDS right before its mode product,
+    // and the products run in place (a wave reads its tiles whole before
+    // writing them; the waves' tiles are disjoint): fewer registers and one
+    // tensor buffer, so more nodes are resident
+    {
+      double b0[4], b1[4];
+      msg_operand8d(sm.Wt[0][0], b0);
+      msg_operand8d(sm.Wt[0][1], b1);
+      mp8dw<D, W, 0, true>(sm.T, sm.T, b0, b1, zl);
     }
-    mp8dw<D, W, 0, true>(sm.T, sm.X, b[0][0], b[0][1], zl);
     __syncthreads();
-    mp8dw<D, W, 1, true>(sm.X, sm.T, b[1][0], b[1][1], zl);
+    {
+      double b0[4], b1[4];
+      msg_operand8d(sm.Wt[1][0], b0);
+      msg_operand8d(sm.Wt[1][1], b1);
+      mp8dw<D, W, 1, true>(sm.T, sm.T, b0, b1, zl);
+    }
     __syncthreads();
-    mp8dw<D, W, 2, true>(sm.T, sm.X, b[2][0], b[2][1], zl);
+    {
+      double b0[4], b1[4];
+      msg_operand8d(sm.Wt[2][0], b0);
+      msg_operand8d(sm.Wt[2][1], b1);
+      mp8dw<D, W, 2, true>(sm.T, sm.T, b0, b1, zl);
+    }
     __syncthreads();
     const double cz = sm.zf[0], sz = sm.zf[1];
     const cplx<double> f0 = mk(cz, -sz), f1 = mk(cz, sz);
-    cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.X);
+    cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.T);
     double ss = 0.0;
     for (int a = t; a < A; a += NT) {
       const cplx<double> t0 = X[a] * f0, t1 = X[A + a] * f1;
