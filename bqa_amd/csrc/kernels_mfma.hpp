@@ -793,7 +793,7 @@ __device__ __forceinline__ void ext_out8x(const float* T, const float* Y, double
 }
 
 template <int D, int W>
-__global__ __launch_bounds__(64 * W) void k_mfma8_ext32(ClassView cv, const cplx<float>* __restrict__ T,
+__global__ __launch_bounds__(64 * W, 6) void k_mfma8_ext32(ClassView cv, const cplx<float>* __restrict__ T,
                                                         const cplx<float>* __restrict__ msgs, cplx<double>* ext_out) {
   static_assert(D == 2 || D == 3, "degree 2 or 3");
   static_assert(W == 1 || W % 2 == 0, "single-p Gram slices per wave");
