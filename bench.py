@@ -206,9 +206,16 @@ def main():
         out = {"summary": s, "run": fast, "ctx": ctx32}
         if want_acc:
             ctx64 = config_to_context(make_config(n, chi, total_steps, "double"))
-            log(f"[bench] chi={chi} fp64 reference run (untimed)")
+            log(f"[bench] chi={chi} fp64 reference run (wall clock only)")
             ref = run_gpu(ctx64, "double", args.warmup, args.steps, timed=False)
             out["accuracy"] = accuracy(fast, ref)
+            s64 = summarize(ref, n, args.steps)
+            out["fp64"] = {"qubit_steps_per_s": round(s64["qubit_steps_per_s"], 2),
+                           "ms_per_step": round(s64["ms_per_step"], 4),
+                           "bp_sweeps_per_step": s64["bp_sweeps_per_step_mean"],
+                           "note": "the FP64 path on the same K steps: the accuracy reference itself"}
+            log(f"[bench] chi={chi} fp64: {s64['qubit_steps_per_s']:.4g} qubit*steps/s, "
+                f"{s64['ms_per_step']:.2f} ms/step")
             log(f"[bench] chi={chi} accuracy {out['accuracy']}")
         return out
 
@@ -238,7 +245,7 @@ def main():
     extra = {}
     if not args.no_chi4 and args.chi != 4:
         c4 = one(4, not args.no_accuracy)
-        extra["chi4"] = dict(c4["summary"], accuracy=c4.get("accuracy"))
+        extra["chi4"] = dict(c4["summary"], accuracy=c4.get("accuracy"), fp64=c4.get("fp64"))
     serving = None
     if args.serve_jobs > 0:
         from bqa_amd.serve import mixed_requests, serve
@@ -289,6 +296,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy": head.get("accuracy"),
+        "fp64": head.get("fp64"),
         "serving": serving,
     }
     result.update(extra)

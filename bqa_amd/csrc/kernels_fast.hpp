@@ -139,6 +139,27 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
   return true;
 }
 
+// FP64, chi = 8, degree 2 or 3: FP64 MFMA (k_mfma8_bp64); degree 3 splits each
+// node over BQA_BP64_WAVES waves.
+#ifndef BQA_BP64_WAVES
+#define BQA_BP64_WAVES 2
+#endif
+
+template <>
+inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
+                                   BpStatus* st, hipStream_t s) {
+  if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
+    return launch_small<kModeBP, double, double>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
+  constexpr int W = BQA_BP64_WAVES;
+  if (cv.d == 3)
+    hipLaunchKernelGGL((k_mfma8_bp64<3, W>), dim3(node_grid<k_mfma8_bp64<3, W>, 64 * W>(cv.n)), dim3(64 * W), 0, s,
+                       cv, T, mA, mB, st);
+  else
+    hipLaunchKernelGGL((k_mfma8_bp64<2, 1>), dim3(node_grid<k_mfma8_bp64<2, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
+                       mA, mB, st);
+  return true;
+}
+
 template <>
 inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>* T, const cplx<float>* msgs,
                                        double* rho, double* bloch, hipStream_t s) {
@@ -149,19 +170,36 @@ inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>*
   return true;
 }
 
+// chi = 8, degree 3 (FP32 and FP64 storage): k_mfma8_local64; other shapes as above.
+template <typename R>
+inline bool fast_local_mfma(ClassView cv, int chi_in, int chi_out, const cplx<R>* Tin, cplx<R>* Tout,
+                            const cplx<double>* canon, int rows, int cols, double zz_time, int flags, double z_time,
+                            double x_time, const double* lmbd, int lstride, cplx<R>* msgs_init, hipStream_t s) {
+  if (chi_in != chi_out || flags != (kAbsorb | kGates | kGauge) || cv.zz == nullptr) return false;
+  if (chi_in != 8 || cv.d != 3 || cv.n == 0)
+    return launch_small_local<R>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init,
+                                 s);
+  hipLaunchKernelGGL((k_mfma8_local64<3, 4, R>), dim3(node_grid<k_mfma8_local64<3, 4, R>, 256>(cv.n)), dim3(256), 0,
+                     s, cv, Tin, Tout, canon, rows, cols, zz_time, z_time, x_time, lmbd, lstride, msgs_init);
+  return true;
+}
+
 template <>
 inline bool fast_local_launch<float>(ClassView cv, int chi_in, int chi_out, const cplx<float>* Tin,
                                      cplx<float>* Tout, const cplx<double>* canon, int rows, int cols,
                                      double zz_time, int flags, double z_time, double x_time, const double* lmbd,
                                      int lstride, cplx<float>* msgs_init, hipStream_t s) {
-  if (chi_in != chi_out || flags != (kAbsorb | kGates | kGauge) || cv.zz == nullptr) return false;
-  if (chi_in != 8 || cv.d != 3 || cv.n == 0)
-    return launch_small_local<float>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride,
-                                     msgs_init, s);
-  hipLaunchKernelGGL((k_mfma8_local64<3, 4, float>), dim3(node_grid<k_mfma8_local64<3, 4, float>, 256>(cv.n)),
-                     dim3(256), 0, s, cv, Tin, Tout, canon, rows, cols, zz_time, z_time, x_time, lmbd, lstride,
-                     msgs_init);
-  return true;
+  return fast_local_mfma<float>(cv, chi_in, chi_out, Tin, Tout, canon, rows, cols, zz_time, flags, z_time, x_time,
+                                lmbd, lstride, msgs_init, s);
+}
+
+template <>
+inline bool fast_local_launch<double>(ClassView cv, int chi_in, int chi_out, const cplx<double>* Tin,
+                                      cplx<double>* Tout, const cplx<double>* canon, int rows, int cols,
+                                      double zz_time, int flags, double z_time, double x_time, const double* lmbd,
+                                      int lstride, cplx<double>* msgs_init, hipStream_t s) {
+  return fast_local_mfma<double>(cv, chi_in, chi_out, Tin, Tout, canon, rows, cols, zz_time, flags, z_time, x_time,
+                                 lmbd, lstride, msgs_init, s);
 }
 
 }  // namespace bqa

@@ -270,11 +270,11 @@ struct Mf8Smem {
 
 // Finish one outgoing BP message from the per-wave Grams (first wave only):
 // the Hermitian part (the message is Hermitian by construction), normalised by
-// its real trace in FP64, stored in FP32 at message position pos, with the
+// its real trace in FP64, stored as R at message position pos, with the
 // partial maxima of |new - old| and |new + old| (old: this lane's entry of the
-// previous message).  Call between the barrier after gram8 and the next one.
-template <int W>
-__device__ __forceinline__ void finish8(const double (*Gp)[256], int pos, cplx<float> old, cplx<float>* mout,
+// previous message).  Call between the barrier after the Gram and the next one.
+template <int W, typename R = float>
+__device__ __forceinline__ void finish8(const double (*Gp)[256], int pos, cplx<R> old, cplx<R>* mout,
                                         double& dnum2, double& dden2) {
   const int l = threadIdx.x & 63, c = l >> 3, e = l & 7;
   // real trace: lanes 0..7 take one diagonal entry each, then a shuffle sum
@@ -283,7 +283,7 @@ __device__ __forceinline__ void finish8(const double (*Gp)[256], int pos, cplx<f
   for (int o = 4; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
   const double inv = 1.0 / __shfl(d, 0, 64);
   const cplx<double> a = gram_entry_w<W>(Gp, c, e), b = gram_entry_w<W>(Gp, e, c);
-  const cplx<float> v = cast<float>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
+  const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
   // squared magnitudes; the square roots are taken once per block
   const cplx<double> vd = cast<double>(v), od = cast<double>(old);
   dnum2 = fmax(dnum2, abs2(vd - od));
@@ -507,11 +507,52 @@ __device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, dou
   }
 }
 
-// Per-lane (and per-wave) parts of the FP64 kernels' LDS indices along each
-// bond; the remaining parts of every index are compile-time constants that
-// add to them (all parts occupy disjoint bits of the natural index), so each
-// access is one base register plus an immediate offset.  Computed once per
-// kernel and passed through opaque() at each use (see Swz8Lanes).
+// LDS double-index swizzle of the FP64 kernels' tensor buffers (natural index
+// 2 x element + component): bit j (j < 5) of a stored index is flipped by the
+// parity of (index & swz64_mask<D>(j)).  Linear over GF(2) like swz, so an
+// index built from disjoint bit fields swizzles field by field.  Every
+// mode-product read and write and every Gram read along each bond touches
+// distinct banks for 64 banks per 32 lanes and for 32 banks per 16 lanes of a
+// 64-bit access (the natural layout had up to 16-way conflicts).  No mask
+// contains bit 0, so the two components of an element stay one aligned pair,
+// swapped where bit 0 is flipped.
+template <int D>
+__host__ __device__ constexpr int swz64_mask(int j) {
+  constexpr int m3[5] = {0x278, 0x6f4, 0x1a8, 0xc0, 0x740};
+  constexpr int m2[5] = {0x48, 0xb4, 0x58, 0xc0, 0xe0};
+  return D == 3 ? m3[j] : m2[j];
+}
+
+template <int D>
+__host__ __device__ constexpr int swz64(int a) {
+  int out = a;
+  for (int j = 0; j < 5; ++j) out ^= (__builtin_popcount(swz64_mask<D>(j) & a) & 1) << j;
+  return out;
+}
+
+__device__ __forceinline__ double lds_ld64(const double* base, int byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ void lds_st64(double* base, int byte_off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// Complex element with swizzled pair index sw = swz64(2 x) of a tensor buffer.
+__device__ __forceinline__ cplx<double> ld_pair64(const double* buf, int sw) {
+  const double2 p = *reinterpret_cast<const double2*>(buf + (sw & ~1));
+  return (sw & 1) ? mk(p.y, p.x) : mk(p.x, p.y);
+}
+
+__device__ __forceinline__ void st_pair64(double* buf, int sw, cplx<double> v) {
+  *reinterpret_cast<double2*>(buf + (sw & ~1)) = (sw & 1) ? make_double2(v.y, v.x) : make_double2(v.x, v.y);
+}
+
+// Per-lane (and per-wave) parts of the FP64 kernels' swizzled LDS indices
+// along each bond, as byte offsets; the remaining parts of every index are
+// compile-time constants on disjoint bits of the natural index, so each access
+// is one XOR with a folded constant.  Computed once per kernel and passed
+// through opaque() at each use (see Swz8Lanes).
 template <int D>
 struct Lin8Lanes {
   int rd[D], wr[D], gl[D];
@@ -523,11 +564,18 @@ __device__ __forceinline__ void lin8_lanes_j(Lin8Lanes<D>& z) {
   constexpr int TPW = Mf8<D>::TILES / W, KPW = Mf8<D>::KS / W;
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
   // A[i][k]: fibre (w TPW + t) 16 + i, element (4 s + kq) & 7 along J, component (4 s + kq) >> 3
-  z.rd[J] = 2 * (fbase8<D, J>(w * TPW * 16 + i16) + kq * st);
+  z.rd[J] = 8 * swz64<D>(2 * (fbase8<D, J>(w * TPW * 16 + i16) + kq * st));
   // D[kq + 4 v][i] (FP64 layout): fibre (w TPW + t) 16 + kq + 4 v, element i & 7, component i >> 3
-  z.wr[J] = 2 * (fbase8<D, J>(w * TPW * 16 + kq) + (i16 & 7) * st) + (i16 >> 3);
+  z.wr[J] = 8 * swz64<D>(2 * (fbase8<D, J>(w * TPW * 16 + kq) + (i16 & 7) * st) + (i16 >> 3));
   // Gram operands: fibre 4 (w KPW + s) + kq, element i & 7 along J, component i >> 3
-  z.gl[J] = 2 * (fbase8<D, J>(4 * w * KPW + kq) + (i16 & 7) * st) + (i16 >> 3);
+  z.gl[J] = 8 * swz64<D>(2 * (fbase8<D, J>(4 * w * KPW + kq) + (i16 & 7) * st) + (i16 >> 3));
+}
+
+// Element x = t + NT k of a tensor buffer (t < NT, NT a power of two): the
+// swizzled pair index from the thread's part and the loop constant.
+template <int D, int NT>
+__device__ __forceinline__ int pair64(int t_sw, int k) {
+  return t_sw ^ swz64<D>(2 * NT * k);
 }
 
 template <int D, int W>
@@ -554,7 +602,8 @@ __device__ __forceinline__ void mp8dw(const double* X, double* Y, const double (
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-    for (int s = 0; s < 4; ++s) a[tt][s] = X[rd + 2 * (fbase8<D, J>(tt * 16) + 4 * (s & 1) * st) + (s >> 1)];
+    for (int s = 0; s < 4; ++s)
+      a[tt][s] = lds_ld64(X, rd ^ (8 * swz64<D>(2 * (fbase8<D, J>(tt * 16) + 4 * (s & 1) * st) + (s >> 1))));
   d4v acc[TPW];
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt) acc[tt] = d4v{0.0, 0.0, 0.0, 0.0};
@@ -568,7 +617,7 @@ __device__ __forceinline__ void mp8dw(const double* X, double* Y, const double (
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Y[wr + 2 * fbase8<D, J>(tt * 16 + 4 * v)] = acc[tt][v];
+    for (int v = 0; v < 4; ++v) lds_st64(Y, wr ^ (8 * swz64<D>(2 * fbase8<D, J>(tt * 16 + 4 * v))), acc[tt][v]);
 }
 
 // This wave's k-steps of the p-resolved Gram along bond J (fibres ordered with
@@ -588,9 +637,9 @@ __device__ __forceinline__ void gram8dw(const double* T, const double* Y, d4v& g
     double ta[B], ya[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-      const int idx = gl + 2 * fbase8<D, J>(4 * (s0 + b));
-      ta[b] = T[idx];
-      ya[b] = Y[idx];
+      const int off = gl ^ (8 * swz64<D>(2 * fbase8<D, J>(4 * (s0 + b))));
+      ta[b] = lds_ld64(T, off);
+      ya[b] = lds_ld64(Y, off);
     }
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -699,10 +748,13 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx
   const int n = cv.n;
   constexpr int S = Mf8<D>::S;
   const Lin8Lanes<D> zl = lin8_lanes<D, W>();
+  static_assert(S % (64 * W) == 0, "tensor load in whole rounds");
+  const int tsw = swz64<D>(2 * (int)threadIdx.x);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const cplx<R>* Tg = T + (size_t)i * S;
-    cplx<double>* Ts = reinterpret_cast<cplx<double>*>(sm.T);
-    for (int x = threadIdx.x; x < S; x += 64 * W) Ts[x] = cast<double>(Tg[x]);
+#pragma unroll
+    for (int k = 0; k < S / (64 * W); ++k)
+      st_pair64(sm.T, pair64<D, 64 * W>(tsw, k), cast<double>(Tg[threadIdx.x + 64 * W * k]));
     double bm[D][4];
 #pragma unroll
     for (int j = 0; j < D; ++j) msg_operand8d(msgs + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
@@ -728,6 +780,118 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx
       mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);
       __syncthreads();
       ext_out8<D, W, 2>(sm.T, sm.W1, sm.W1, cv, i, ext_out, zl);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BP sweep of an FP64 job, chi = 8, degree 2 or 3: the FP32 kernel's schedule
+// (one work buffer, mode products in place along their bond, T x2 m2 computed
+// twice for degree 3) on v_mfma_f64_16x16x4f64 over the natural layout of the
+// FP64 extended-message kernel, one block of W waves per node.  Each wave
+// stores its Gram partial (both p halves summed) in its own slot; the first
+// wave finishes the message while the others start the next mode product,
+// whose barrier keeps the partials alive until it has read them.
+template <int D, int W>
+struct Mf8Bp64Smem {
+  double T[2 * Mf8<D>::S];
+  double W1[2 * Mf8<D>::S];
+  double Gp[W][256];
+};
+
+// This wave's Gram partial along bond J into its slot of Gp (FP64 D layout).
+template <int D, int W, int J>
+__device__ __forceinline__ void gram8_bp64(const double* T, const double* Y, double (*Gp)[256],
+                                           const Lin8Lanes<D>& z) {
+  d4v g0, g1;
+  gram8dw<D, W, J>(T, Y, g0, g1, z);
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Gp[w][(kq + 4 * v) * 16 + i16] = g0[v] + g1[v];
+}
+
+template <int D, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma8_bp64(ClassView cv, const cplx<double>* __restrict__ T,
+                                                       cplx<double>* msgA, cplx<double>* msgB, BpStatus* st) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(Mf8<D>::TILES % W == 0 && Mf8<D>::KS % W == 0, "tiles and k-steps split evenly over the waves");
+  static_assert(Mf8<D>::S % (64 * W) == 0, "tensor load in whole rounds");
+  __shared__ __align__(16) Mf8Bp64Smem<D, W> sm;
+  if (st->done) return;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool lead = W == 1 || wave == 0;
+  const int cur = st->cur;
+  const cplx<double>* min = cur ? msgB : msgA;
+  cplx<double>* mout = cur ? msgA : msgB;
+  double dnum = 0.0, dden = 0.0;
+  constexpr int S = Mf8<D>::S;
+  const Lin8Lanes<D> zl = lin8_lanes<D, W>();
+  const int tsw = swz64<D>(2 * (int)threadIdx.x);
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<double>* Tg = T + (size_t)i * S;
+#pragma unroll
+    for (int k = 0; k < S / (64 * W); ++k) st_pair64(sm.T, pair64<D, 64 * W>(tsw, k), Tg[threadIdx.x + 64 * W * k]);
+    double bm[D][4];
+#pragma unroll
+    for (int j = 0; j < D; ++j) msg_operand8d(min + (size_t)cv.in_pos[j * n + i] * 64, bm[j]);
+    int pos[D];
+    cplx<double> old[D];
+    if (lead) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        pos[k] = cv.out_pos[k * n + i];
+        old[k] = min[(size_t)pos[k] * 64 + l];
+      }
+    }
+    node_sync<W>();
+    if constexpr (D == 2) {
+      mp8dw<D, W, 1, false>(sm.T, sm.W1, bm[1], bm[1], zl);   // Y0 = T x1 m1
+      node_sync<W>();
+      gram8_bp64<D, W, 0>(sm.T, sm.W1, sm.Gp, zl);
+      node_sync<W>();
+      if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
+      mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);   // Y1 = T x0 m0
+      node_sync<W>();
+      gram8_bp64<D, W, 1>(sm.T, sm.W1, sm.Gp, zl);
+      node_sync<W>();
+      if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
+    } else {
+      mp8dw<D, W, 2, false>(sm.T, sm.W1, bm[2], bm[2], zl);   // W1 = T x2 m2
+      node_sync<W>();
+      mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);  // Y0 = W1 x1 m1
+      node_sync<W>();
+      gram8_bp64<D, W, 0>(sm.T, sm.W1, sm.Gp, zl);
+      node_sync<W>();
+      if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
+      mp8dw<D, W, 2, false>(sm.T, sm.W1, bm[2], bm[2], zl);   // W1 = T x2 m2
+      node_sync<W>();
+      mp8dw<D, W, 0, false>(sm.W1, sm.W1, bm[0], bm[0], zl);  // Y1 = W1 x0 m0
+      node_sync<W>();
+      gram8_bp64<D, W, 1>(sm.T, sm.W1, sm.Gp, zl);
+      node_sync<W>();
+      if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
+      mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);   // W1 = T x0 m0
+      node_sync<W>();
+      mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);  // Y2 = W1 x1 m1
+      node_sync<W>();
+      gram8_bp64<D, W, 2>(sm.T, sm.W1, sm.Gp, zl);
+      node_sync<W>();
+      if (lead) finish8<W, double>(sm.Gp, pos[2], old[2], mout, dnum, dden);
+    }
+    // the next node's tensor load overwrites T, which the last Gram read
+    node_sync<W>();
+  }
+  if (lead) {
+    for (int o = 32; o > 0; o >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+      dden = fmax(dden, __shfl_xor(dden, o, 64));
+    }
+    dnum = sqrt(dnum);  // the partial maxima were of squared magnitudes
+    dden = sqrt(dden);
+    if (l == 0) {
+      atomic_max_nonneg(&st->num_bits, dnum);
+      atomic_max_nonneg(&st->den_bits, dden);
     }
   }
 }
@@ -873,10 +1037,12 @@ __global__ __launch_bounds__(64 * W, 5) void k_mfma8_local64(ClassView cv, const
   const int t = threadIdx.x, l = t & 63, wave = t >> 6;
   const double cx = cos(x_time), sx = sin(x_time);
   const Lin8Lanes<D> zl = lin8_lanes<D, W>();
+  static_assert(A % NT == 0, "tensor passes in whole rounds");
+  const int tsw = swz64<D>(2 * t);
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const cplx<R>* Tg = Tin + (size_t)i * S;
-    cplx<double>* Ts = reinterpret_cast<cplx<double>*>(sm.T);
-    for (int x = t; x < S; x += NT) Ts[x] = cast<double>(Tg[x]);
+#pragma unroll
+    for (int k = 0; k < S / NT; ++k) st_pair64(sm.T, pair64<D, NT>(tsw, k), cast<double>(Tg[t + NT * k]));
     for (int q = t; q < D * 64; q += NT) {
       const int k = q >> 6, nn = (q >> 3) & 7, c = q & 7;
       const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
@@ -920,15 +1086,17 @@ __global__ __launch_bounds__(64 * W, 5) void k_mfma8_local64(ClassView cv, const
     __syncthreads();
     const double cz = sm.zf[0], sz = sm.zf[1];
     const cplx<double> f0 = mk(cz, -sz), f1 = mk(cz, sz);
-    cplx<double>* X = reinterpret_cast<cplx<double>*>(sm.T);
     double ss = 0.0;
-    for (int a = t; a < A; a += NT) {
-      const cplx<double> t0 = X[a] * f0, t1 = X[A + a] * f1;
+#pragma unroll
+    for (int k = 0; k < A / NT; ++k) {
+      const int a = t + NT * k;
+      const int s0 = pair64<D, NT>(tsw, k), s1 = s0 ^ swz64<D>(2 * A);  // elements a and A + a
+      const cplx<double> t0 = ld_pair64(sm.T, s0) * f0, t1 = ld_pair64(sm.T, s1) * f1;
       const double f = sm.sl[0][a >> 6] * sm.sl[1][(a >> 3) & 7] * sm.sl[2][a & 7];
       const cplx<double> y0 = mk(f * (cx * t0.x + sx * t1.y), f * (cx * t0.y - sx * t1.x));
       const cplx<double> y1 = mk(f * (cx * t1.x + sx * t0.y), f * (cx * t1.y - sx * t0.x));
-      X[a] = y0;
-      X[A + a] = y1;
+      st_pair64(sm.T, s0, y0);
+      st_pair64(sm.T, s1, y1);
       ss += abs2(y0) + abs2(y1);
     }
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
@@ -939,7 +1107,8 @@ __global__ __launch_bounds__(64 * W, 5) void k_mfma8_local64(ClassView cv, const
     for (int w = 0; w < W; ++w) tot += sm.red[w];
     const double inv = 1.0 / sqrt(tot);
     cplx<R>* To = Tout + (size_t)i * S;
-    for (int x = t; x < S; x += NT) To[x] = cast<R>(scale(X[x], inv));
+#pragma unroll
+    for (int k = 0; k < S / NT; ++k) To[t + NT * k] = cast<R>(scale(ld_pair64(sm.T, pair64<D, NT>(tsw, k)), inv));
     for (int q = t; q < D * 64; q += NT) {
       const int k = q >> 6, a = (q >> 3) & 7, bcol = q & 7;
       const double* lam = lmbd + (size_t)cv.lmbd_pos[k * n + i] * lstride;
