@@ -282,7 +282,10 @@ __device__ __forceinline__ void finish8(const double (*Gp)[256], int pos, cplx<R
 #pragma unroll
   for (int o = 4; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
   const double inv = 1.0 / __shfl(d, 0, 64);
-  const cplx<double> a = gram_entry_w<W>(Gp, c, e), b = gram_entry_w<W>(Gp, e, c);
+  // entry (e, c) from its own lane: the transposed LDS reads had 4-way bank conflicts
+  const cplx<double> a = gram_entry_w<W>(Gp, c, e);
+  const int tl = e * 8 + c;
+  const cplx<double> b = mk(__shfl(a.x, tl, 64), __shfl(a.y, tl, 64));
   const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
   // squared magnitudes; the square roots are taken once per block
   const cplx<double> vd = cast<double>(v), od = cast<double>(old);
