@@ -275,8 +275,14 @@ def main():
         cap = serve(reqs, max_concurrent=8)
         serving["capacity"] = {k: v for k, v in keep(cap).items() if k in ("jobs", "failed", "makespan_s", "jobs_per_h",
                                                                         "qubit_steps_per_s", "p50_s", "p99_s")}
+        # the same queued jobs one at a time (the plain FP32 path, no overlap)
+        seq = serve(reqs, max_concurrent=1)
+        serving["one_at_a_time"] = {k: v for k, v in keep(seq).items() if k in ("jobs", "failed", "makespan_s",
+                                                                               "jobs_per_h", "p50_s", "p99_s")}
+        serving["capacity_over_one_at_a_time"] = round(cap["jobs_per_h"] / max(1e-12, seq["jobs_per_h"]), 3)
         log(f"[bench] serving: {serving['jobs_per_h']:.4g} jobs/h offered, p50 {serving['p50_s']:.3f} s, "
-            f"p99 {serving['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h; failed {serving['failed']}")
+            f"p99 {serving['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h; failed {serving['failed']}; "
+            f"one at a time {seq['jobs_per_h']:.4g} jobs/h ({serving['capacity_over_one_at_a_time']:.2f}x)")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (FP64 NumPy oracle)")
