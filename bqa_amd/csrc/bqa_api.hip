@@ -948,7 +948,7 @@ int bqa_job_destroy(bqa_job* j) {
       c.T[1].release(j->dev_bytes);
     }
     DevBuf* bufs[] = {&j->msgs[0], &j->msgs[1], &j->lmbd, &j->ext, &j->canon, &j->lmax, &j->status,
-                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws, &j->conv, &j->conv_msgs};
+                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws, &j->conv, &j->conv_msgs, &j->zz};
     for (DevBuf* b : bufs) b->release(j->dev_bytes);
     for (auto& pr : j->ev_pending) {
       (void)hipEventDestroy(pr.first);
