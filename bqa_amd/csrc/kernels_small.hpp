@@ -22,6 +22,16 @@ namespace bqa {
 
 constexpr int cpow(int b, int e) { return e <= 0 ? 1 : b * cpow(b, e - 1); }
 
+// Phase boundary of these one-wave blocks (every launch here is 64 threads):
+// the wave's LDS operations complete in issue order, so wavefront-scope fences
+// around a wave barrier order the accesses without a hardware barrier and its
+// wait for every outstanding LDS operation.
+__device__ __forceinline__ void small_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int D, int CHI>
 struct Sm {
   static constexpr int A = cpow(CHI, D);   // chi^d
@@ -100,7 +110,7 @@ __device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<C
   } else {
 #pragma unroll
     for (int p = 0; p < (PSPLIT ? 2 : 1); ++p) red[p][l] = g[p];
-    __syncthreads();
+    small_sync();
     if (l < M::Q) {
 #pragma unroll
       for (int p = 0; p < (PSPLIT ? 2 : 1); ++p) {
@@ -110,7 +120,7 @@ __device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<C
         g[p] = s;
       }
     }
-    __syncthreads();
+    small_sync();
   }
 }
 
@@ -122,7 +132,7 @@ __device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)
   constexpr int Q = CHI * CHI;
   const int l = threadIdx.x & 63;
   if (l < Q) red[0][l] = g[0];
-  __syncthreads();
+  small_sync();
   if (l < Q) {
     const int c = l / CHI, e = l % CHI;
     double tr = 0.0;
@@ -135,7 +145,7 @@ __device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)
     dden = fmax(dden, (double)cabs(v + old));
     mout[pos * Q + l] = v;
   }
-  __syncthreads();
+  small_sync();
 }
 
 // Extended message (2chi x 2chi, FP64) at position pos from the p-resolved Gram
@@ -149,7 +159,7 @@ __device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<dou
     red[0][l] = g[0];
     red[1][l] = g[1];
   }
-  __syncthreads();
+  small_sync();
   cplx<double> tr = mk(0.0, 0.0);
 #pragma unroll
   for (int k = 0; k < CHI; ++k) tr = tr + red[0][k * CHI + k] + red[1][k * CHI + k];
@@ -166,7 +176,7 @@ __device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<dou
       ext_out[pos * N2 * N2 + o] = conj(ga) * gb * gsum * inv;
     }
   }
-  __syncthreads();
+  small_sync();
 }
 
 // Outgoing message (BP) or extended message (Ext) along bond K from Y_K.
@@ -221,21 +231,21 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
 #pragma unroll
     for (int k = 0; k < D; ++k)
       old[k] = (MODE == kModeBP && l < M::Q) ? min[(size_t)cv.out_pos[k * n + i] * M::Q + l] : mk<R>(0, 0);
-    __syncthreads();
+    small_sync();
     if constexpr (MODE == kModeDensity) {
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);
-      __syncthreads();
+      small_sync();
       const cplx<C>* Yf = sm.Y;
       if constexpr (D >= 3) {
         smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);
-        __syncthreads();
+        small_sync();
         Yf = sm.X;
       }
       if constexpr (D == 4) {
         smp<D, CHI, 3, C>(sm.X, sm.Y, sm.m[3]);
-        __syncthreads();
+        small_sync();
         Yf = sm.Y;
       }
       // rho[p][q] = sum_a Y[p, a] conj(T[q, a])
@@ -273,51 +283,51 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
           bo[2] = r00.x - r11.x;
         }
       }
-      __syncthreads();
+      small_sync();
       continue;
     } else if constexpr (D == 2) {
       smp<D, CHI, 1, C>(sm.T, sm.X, sm.m[1]);  // Y0 = T x1 m1
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // Y1 = T x0 m0
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     } else if constexpr (D == 3) {
       smp<D, CHI, 2, C>(sm.T, sm.X, sm.m[2]);  // W = T x2 m2
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y0 = W x1 m1
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.X, sm.Y, sm.m[0]);  // Y1 = W x0 m0
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // W = T x0 m0
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y2 = W x1 m1
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
     } else {
       // degree 4: W = T x3 m3 x2 m2 shared by outputs 0 and 1, V = T x0 m0 x1 m1
       // shared by outputs 2 and 3 (8 mode products instead of 12).
       smp<D, CHI, 3, C>(sm.T, sm.X, sm.m[3]);
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 2, C>(sm.X, sm.Y, sm.m[2]);  // W in Y
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 1, C>(sm.Y, sm.X, sm.m[1]);  // Y0 = W x1 m1
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.Y, sm.X, sm.m[0]);  // Y1 = W x0 m0
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // V in Y
-      __syncthreads();
+      small_sync();
       smp<D, CHI, 3, C>(sm.Y, sm.X, sm.m[3]);  // Y2 = V x3 m3
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 2, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);  // Y3 = V x2 m2
-      __syncthreads();
+      small_sync();
       sout<D, CHI, 3, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     }
   }
@@ -452,20 +462,20 @@ __global__ __launch_bounds__(64) void k_small_local(ClassView cv, const cplx<R>*
       sm.red[0] = cos(phi);
       sm.red[1] = sin(phi);
     }
-    __syncthreads();
+    small_sync();
     smp_absorb<D, CHI, 0>(sm.X, sm.Y, sm.W[0]);
-    __syncthreads();
+    small_sync();
     smp_absorb<D, CHI, 1>(sm.Y, sm.X, sm.W[1]);
-    __syncthreads();
+    small_sync();
     cplx<double>* F = sm.X;
     if constexpr (D >= 3) {
       smp_absorb<D, CHI, 2>(sm.X, sm.Y, sm.W[2]);
-      __syncthreads();
+      small_sync();
       F = sm.Y;
     }
     if constexpr (D == 4) {
       smp_absorb<D, CHI, 3>(sm.Y, sm.X, sm.W[3]);
-      __syncthreads();
+      small_sync();
       F = sm.X;
     }
     // Z then X on the physical index, then sqrt(lambda) on every bond and the L2 norm
@@ -489,7 +499,7 @@ __global__ __launch_bounds__(64) void k_small_local(ClassView cv, const cplx<R>*
     }
     for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
     const double inv = 1.0 / sqrt(ss);
-    __syncthreads();
+    small_sync();
     cplx<R>* To = Tout + (size_t)i * M::S;
 #pragma unroll 4
     for (int q = 0; q < (M::S + 63) / 64; ++q) {
@@ -504,7 +514,7 @@ __global__ __launch_bounds__(64) void k_small_local(ClassView cv, const cplx<R>*
       for (int e = 0; e < CHI; ++e) tr += lam[e];
       msgs_init[(size_t)cv.in_pos[k * n + i] * M::Q + o] = mk<R>(a == b ? (R)(lam[a] / tr) : (R)0, (R)0);
     }
-    __syncthreads();
+    small_sync();
   }
 }
 
