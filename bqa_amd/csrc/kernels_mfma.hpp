@@ -495,8 +495,10 @@ __device__ __forceinline__ d4v mfma4d(double a, double b, d4v c) {
 }
 
 // FP64 16x16x4 MFMA lane layout (checked on gfx950): A[i][k] and B[k][j] as in
-// FP32, D[l / 16 + 4 v][l % 16] in result register v.
-template <typename R>
+// FP32, D[l / 16 + 4 v][l % 16] in result register v.  RS: the row stride of m
+// in complex elements (8 for a message; 9 for an LDS table, where rows 8
+// elements apart would put the 8 row reads of a k-step on 2 bank groups).
+template <int RS = 8, typename R>
 __device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, double (&bm)[4]) {
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4;
   const int a = i16 & 7, cn = i16 >> 3;
@@ -504,7 +506,7 @@ __device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, dou
   for (int s = 0; s < 4; ++s) {
     const int kr = 4 * s + kq;
     const int b = kr & 7, ck = kr >> 3;
-    const cplx<R> v = m[a * 8 + b];
+    const cplx<R> v = m[a * RS + b];
     const double vr = (double)v.x, vi = (double)v.y;
     bm[s] = ck == 0 ? (cn == 0 ? vr : vi) : (cn == 0 ? -vi : vr);
   }
@@ -1019,7 +1021,7 @@ namespace bqa {
 template <int D>
 struct Mf8LocalSmemD {
   double T[2 * Mf8<D>::S];  // the mode products run in place
-  cplx<double> Wt[D][2][64];
+  cplx<double> Wt[D][2][72];  // [bond][p][n * 9 + c]: rows padded to 9 elements
   double sl[D][8];
   double red[4];
   double zf[2];  // cos, sin of the node's Z-gate angle
@@ -1051,8 +1053,8 @@ __global__ __launch_bounds__(64 * W, 5) void k_mfma8_local64(ClassView cv, const
       const zd* C = canon + (size_t)cv.in_pos[k * n + i] * canon_rows * canon_cols;
       const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
       const cplx<double> u = zg[0] * C[c * canon_cols + nn], v = zg[1] * C[(8 + c) * canon_cols + nn];
-      sm.Wt[k][0][nn * 8 + c] = u + v;
-      sm.Wt[k][1][nn * 8 + c] = u - v;
+      sm.Wt[k][0][nn * 9 + c] = u + v;
+      sm.Wt[k][1][nn * 9 + c] = u - v;
       if (c == 0 && nn == 0)
         for (int e = 0; e < 8; ++e) sm.sl[k][e] = sqrt(lmbd[(size_t)cv.lmbd_pos[k * n + i] * lstride + e]);
     }
@@ -1068,22 +1070,22 @@ __global__ __launch_bounds__(64 * W, 5) void k_mfma8_local64(ClassView cv, const
     // tensor buffer, so more nodes are resident
     {
       double b0[4], b1[4];
-      msg_operand8d(sm.Wt[0][0], b0);
-      msg_operand8d(sm.Wt[0][1], b1);
+      msg_operand8d<9>(sm.Wt[0][0], b0);
+      msg_operand8d<9>(sm.Wt[0][1], b1);
       mp8dw<D, W, 0, true>(sm.T, sm.T, b0, b1, zl);
     }
     __syncthreads();
     {
       double b0[4], b1[4];
-      msg_operand8d(sm.Wt[1][0], b0);
-      msg_operand8d(sm.Wt[1][1], b1);
+      msg_operand8d<9>(sm.Wt[1][0], b0);
+      msg_operand8d<9>(sm.Wt[1][1], b1);
       mp8dw<D, W, 1, true>(sm.T, sm.T, b0, b1, zl);
     }
     __syncthreads();
     {
       double b0[4], b1[4];
-      msg_operand8d(sm.Wt[2][0], b0);
-      msg_operand8d(sm.Wt[2][1], b1);
+      msg_operand8d<9>(sm.Wt[2][0], b0);
+      msg_operand8d<9>(sm.Wt[2][1], b1);
       mp8dw<D, W, 2, true>(sm.T, sm.T, b0, b1, zl);
     }
     __syncthreads();
