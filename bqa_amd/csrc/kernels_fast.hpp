@@ -43,7 +43,7 @@ inline int node_grid(int n) {
 }
 
 // Compile-time-shaped one-wave kernels (kernels_small.hpp) for every (d, chi)
-// they serve; the FP32 chi = 8 specialisations below use MFMA instead.
+// they serve; the chi = 8 specialisations below (FP32 and FP64) use MFMA instead.
 template <typename R>
 bool fast_bp_launch(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
                     hipStream_t s) {
@@ -51,7 +51,7 @@ bool fast_bp_launch(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R
 }
 
 // The fused absorb/gates/gauge step at a fixed bond dimension: one-wave
-// kernels (kernels_small.hpp); the FP32 chi = 8, degree-3 case uses MFMA.
+// kernels (kernels_small.hpp); the chi = 8, degree-3 case uses MFMA (both precisions).
 template <typename R>
 bool fast_local_launch(ClassView cv, int chi_in, int chi_out, const cplx<R>* Tin, cplx<R>* Tout,
                        const cplx<double>* canon, int rows, int cols, double, int flags, double z_time, double x_time,

@@ -5,7 +5,8 @@
 // They replace the generic k_node_contract (one 256-thread workgroup per node,
 // runtime index arithmetic, a barrier around every small step) on the shapes
 // that occur while the bond dimension ramps up to chi_max, on chi_max = 4, and
-// on every FP64 job.  Same contractions and results as k_node_contract:
+// on FP64 jobs below chi = 8 (and for their readout at chi = 8).  Same
+// contractions and results as k_node_contract:
 //   BP       (state.py:97-124, backends.py:381-408): new messages, Hermitian
 //            part over its real trace, dist partial maxima;
 //   Ext      (state.py:127-139, backends.py:393-404): 2chi x 2chi extended
