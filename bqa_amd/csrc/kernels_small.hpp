@@ -142,8 +142,10 @@ __device__ __forceinline__ void sfinish_bp(const cplx<C> (&g)[2], cplx<C> (&red)
     const double inv = 1.0 / tr;
     const cplx<double> a = cast<double>(red[0][l]), b = cast<double>(red[0][e * CHI + c]);
     const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
-    dnum = fmax(dnum, (double)cabs(v - old));
-    dden = fmax(dden, (double)cabs(v + old));
+    // squared magnitudes; the square roots are taken once per block
+    const cplx<double> vd = cast<double>(v), od = cast<double>(old);
+    dnum = fmax(dnum, abs2(vd - od));
+    dden = fmax(dden, abs2(vd + od));
     mout[pos * Q + l] = v;
   }
   small_sync();
@@ -338,8 +340,8 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
       dden = fmax(dden, __shfl_xor(dden, o, 64));
     }
     if (l == 0) {
-      atomic_max_nonneg(&st->num_bits, dnum);
-      atomic_max_nonneg(&st->den_bits, dden);
+      atomic_max_nonneg(&st->num_bits, sqrt(dnum));
+      atomic_max_nonneg(&st->den_bits, sqrt(dden));
     }
   }
 }
