@@ -106,6 +106,12 @@ long ipow_l(int b, int e) {
   return r;
 }
 
+double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
 }  // namespace
 
 struct ClassDev {
@@ -126,6 +132,7 @@ struct ClassDev {
     v.h = h;
     v.J = J;
     v.zz = nullptr;
+    v.sweep = -1;
     return v;
   }
 };
@@ -295,6 +302,11 @@ __global__ void k_bp_reset(BpStatus* st, int cur, int max_iters, double bp_eps) 
   st->have_best = 0;
   st->damp_now = 0;
   st->max_iters = max_iters;
+  st->marker = 0;
+  for (int k = 0; k < kSweepSlots && k < max_iters; ++k) {
+    st->slot[k][0] = 0ULL;
+    st->slot[k][1] = 0ULL;
+  }
   st->bp_eps = bp_eps;
   st->last_dist = 0.0;
   st->best_dist = INFINITY;
@@ -403,8 +415,9 @@ int grid_for(long n, int block = 256, int cap = 8192) {
 
 // ---------------------------------------------------------------- phases --
 
+// One class's launch of a BP sweep; sweep >= 0 selects the sweep-slot protocol.
 template <typename R>
-void launch_bp_class(bqa_job* j, ClassDev& c) {
+void launch_bp_class(bqa_job* j, ClassDev& c, int sweep) {
   const int chi = j->chi;
   if (c.d == 0 || c.n == 0) return;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -413,12 +426,14 @@ void launch_bp_class(bqa_job* j, ClassDev& c) {
     e1 = take_event(j);
     HIPX(hipEventRecord(e0, j->stream));
   }
-  if (j->generic(kGenBP) || !fast_bp_launch<R>(c.view(), chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
+  ClassView view = c.view();
+  view.sweep = sweep;
+  if (j->generic(kGenBP) || !fast_bp_launch<R>(view, chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
                          msg_ptr<R>(j, 1), reinterpret_cast<BpStatus*>(j->status.p), j->stream)) {
     const long S = 2 * ipow_l(chi, c.d);
     const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
     NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 3 * S, fixed, 3 * S, sizeof(cplx<R>));
-    hipLaunchKernelGGL((k_node_contract<R, kModeBP>), dim3(L.grid), dim3(kNT), L.lds, j->stream, c.view(), chi,
+    hipLaunchKernelGGL((k_node_contract<R, kModeBP>), dim3(L.grid), dim3(kNT), L.lds, j->stream, view, chi,
                        reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0), msg_ptr<R>(j, 1),
                        reinterpret_cast<BpStatus*>(j->status.p), (cplx<R>*)nullptr, 0.0, (double*)nullptr,
                        (double*)nullptr, reinterpret_cast<cplx<R>*>(L.ws), L.ws_elems);
@@ -444,12 +459,19 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
   const size_t ev_start = j->ev_pending.size();
   int active_classes = 0;
   for (auto& cl : j->cls) active_classes += (cl.d > 0 && cl.n > 0) ? 1 : 0;
-  int launched = 0;
+  // Without damping, sweeps run on the sweep-slot protocol: no kernel between
+  // sweeps, the rules applied here to each chunk's slots.
+  const bool slots = !damping_on && active_classes > 0 && max_iters <= kSweepSlots;
+  BpStatus h{};
+  h.cur = j->mcur;
+  h.best_dist = INFINITY;
+  int launched = 0, ruled = 0;
   int chunk = std::max(1, std::min(j->pred_sweeps, max_iters));
   while (launched < max_iters) {
     const int c = std::min(chunk, max_iters - launched);
     for (int s = 0; s < c; ++s) {
-      for (auto& cl : j->cls) launch_bp_class<R>(j, cl);
+      for (auto& cl : j->cls) launch_bp_class<R>(j, cl, slots ? launched + s : -1);
+      if (slots) continue;
       hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, j->stream, st, damping_on);
       LAUNCH_CHECK();
       if (damping_on) {
@@ -461,12 +483,37 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
     launched += c;
     HIPX(hipMemcpyAsync(j->st_host, st, sizeof(BpStatus), hipMemcpyDeviceToHost, j->stream));
     HIPX(hipStreamSynchronize(j->stream));
-    if (j->st_host->done) break;
+    if (slots) {
+      // the rules of k_bp_finish (no damping) over the sweeps of this chunk
+      for (; ruled < launched && !h.done; ++ruled) {
+        const double num = bits_to_double(j->st_host->slot[ruled][0]);
+        const double den = bits_to_double(j->st_host->slot[ruled][1]);
+        const double dist = num / den;
+        h.iters += 1;
+        h.last_dist = dist;
+        if (dist < h.best_dist) {
+          h.best_dist = dist;
+          h.have_best = 1;
+        }
+        if (dist < bp_eps) {
+          h.converged = 1;
+          h.done = 1;  // keep the previous iterate
+        } else if (h.iters >= max_iters) {
+          h.done = 1;
+          h.cur = 1 - h.cur;  // the last undamped iterate
+        } else {
+          h.cur = 1 - h.cur;
+        }
+      }
+    } else {
+      h = *j->st_host;
+    }
+    if (h.done) break;
     chunk = 4;
   }
   if (j->timing) {
     // launches after convergence exit at once: time only the sweeps that ran
-    const size_t keep = ev_start + (size_t)j->st_host->iters * active_classes;
+    const size_t keep = ev_start + (size_t)h.iters * active_classes;
     while (j->ev_pending.size() > keep) {
       j->ev_pool.push_back(j->ev_pending.back().first);
       j->ev_pool.push_back(j->ev_pending.back().second);
@@ -475,7 +522,6 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
     }
     flush_timing(j);
   }
-  const BpStatus& h = *j->st_host;
   j->mcur = h.cur;
   j->pred_sweeps = std::max(1, h.iters);
   *sweeps = h.iters;

@@ -132,22 +132,54 @@ __host__ __device__ inline int ipow(int b, int e) {
   return r;
 }
 
-// BP control block living in device memory.  The sweep kernels read `cur` and
-// `done`; the finish kernel (one thread) updates them after every sweep.
+constexpr int kSweepSlots = 128;
+
+// BP control block living in device memory.  Two protocols:
+//  - status (sweep index < 0; damping runs): the sweep kernels read `cur` and
+//    `done`, and the finish kernel (one thread) updates them after every sweep;
+//  - sweep slots (sweep index >= 0): each sweep writes its distance maxima to
+//    its own slot, each launch decides from the previous sweep's slot whether
+//    the run is over, and the host applies the rules to the slots after every
+//    chunk of sweeps, so no kernel runs between sweeps.
 struct BpStatus {
   unsigned long long num_bits;  // max |new - old| as double bits
   unsigned long long den_bits;  // max |new + old|
-  int cur;                      // index of the current message buffer
+  int cur;                      // index of the current message buffer (slots: of the run's first sweep)
   int done;                     // 1 once converged or out of iterations
   int converged;
   int iters;                    // sweeps performed
   int have_best;                // some sweep gave a finite distance
   int damp_now;                 // the damping kernel must run after this sweep
   int max_iters;
-  int pad;
+  int marker;                   // slots: a launch found the run finished before its sweep
   double bp_eps;
   double last_dist;
   double best_dist;
+  unsigned long long slot[kSweepSlots][2];  // slots: the two maxima of each sweep
 };
+
+// Buffer index a BP launch of sweep `sweep` reads, or -1 when the run is over.
+// With slots, a sweep runs unless an earlier one converged (the previous
+// sweep's distance below bp_eps, or a launch before it said so), and the
+// buffers alternate from the run's first one.
+__device__ inline int bp_begin(BpStatus* st, int sweep) {
+  if (sweep < 0) return st->done ? -1 : st->cur;
+  if (sweep > 0) {
+    if (st->marker) return -1;
+    const double num = __longlong_as_double((long long)st->slot[sweep - 1][0]);
+    const double den = __longlong_as_double((long long)st->slot[sweep - 1][1]);
+    if (num / den < st->bp_eps) {
+      if (threadIdx.x == 0) st->marker = 1;
+      return -1;
+    }
+  }
+  return st->cur ^ (sweep & 1);
+}
+
+// This block's distance maxima of sweep `sweep` (one thread per block).
+__device__ inline void bp_end(BpStatus* st, int sweep, double dnum, double dden) {
+  atomic_max_nonneg(sweep < 0 ? &st->num_bits : &st->slot[sweep][0], dnum);
+  atomic_max_nonneg(sweep < 0 ? &st->den_bits : &st->slot[sweep][1], dden);
+}
 
 }  // namespace bqa

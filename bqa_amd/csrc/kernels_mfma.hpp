@@ -335,8 +335,8 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
   cplx<float>* min = msgA;
   cplx<float>* mout = msgB;
   if (MODE == kModeBP) {
-    if (st->done) return;
-    const int cur = st->cur;
+    const int cur = bp_begin(st, cv.sweep);
+    if (cur < 0) return;
     min = cur ? msgB : msgA;
     mout = cur ? msgA : msgB;
   }
@@ -469,8 +469,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
     dnum = sqrt(dnum);  // the partial maxima were of squared magnitudes
     dden = sqrt(dden);
     if (l == 0) {
-      atomic_max_nonneg(&st->num_bits, dnum);
-      atomic_max_nonneg(&st->den_bits, dden);
+      bp_end(st, cv.sweep, dnum, dden);
     }
   }
 }
@@ -822,11 +821,11 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_bp64(ClassView cv, const cplx<
   static_assert(Mf8<D>::TILES % W == 0 && Mf8<D>::KS % W == 0, "tiles and k-steps split evenly over the waves");
   static_assert(Mf8<D>::S % (64 * W) == 0, "tensor load in whole rounds");
   __shared__ __align__(16) Mf8Bp64Smem<D, W> sm;
-  if (st->done) return;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
   const int n = cv.n;
   const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool lead = W == 1 || wave == 0;
-  const int cur = st->cur;
   const cplx<double>* min = cur ? msgB : msgA;
   cplx<double>* mout = cur ? msgA : msgB;
   double dnum = 0.0, dden = 0.0;
@@ -895,8 +894,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_bp64(ClassView cv, const cplx<
     dnum = sqrt(dnum);  // the partial maxima were of squared magnitudes
     dden = sqrt(dden);
     if (l == 0) {
-      atomic_max_nonneg(&st->num_bits, dnum);
-      atomic_max_nonneg(&st->den_bits, dden);
+      bp_end(st, cv.sweep, dnum, dden);
     }
   }
 }

@@ -24,6 +24,9 @@ struct ClassView {
   // ZZ-split gates of the current simple update per edge (lambda position e):
   // g0 = zz[2 e], g1 = zz[2 e + 1] (zz_split of J_e ztime); nullptr outside it
   const cplx<double>* zz;
+  // BP launches: the sweep's index in its run with the sweep-slot protocol,
+  // -1 for the status protocol (BpStatus)
+  int sweep;
 };
 
 enum NodeMode { kModeBP = 0, kModeExt = 1, kModeDensity = 2 };
@@ -127,8 +130,8 @@ __global__ __launch_bounds__(kNT) void k_node_contract(
   const cplx<R>* min = msgA;
   cplx<R>* mout = msgB;
   if (MODE == kModeBP) {
-    if (st->done) return;
-    const int cur = st->cur;
+    const int cur = bp_begin(st, cv.sweep);
+    if (cur < 0) return;
     min = cur ? msgB : msgA;
     mout = cur ? msgA : msgB;
   }
@@ -245,8 +248,7 @@ __global__ __launch_bounds__(kNT) void k_node_contract(
     dnum = block_max(dnum, shred);
     dden = block_max(dden, shred);
     if (threadIdx.x == 0) {
-      atomic_max_nonneg(&st->num_bits, dnum);
-      atomic_max_nonneg(&st->den_bits, dden);
+      bp_end(st, cv.sweep, dnum, dden);
     }
   }
 }

@@ -211,8 +211,8 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
   const cplx<R>* min = msgA;
   cplx<R>* mout = msgB;
   if (MODE == kModeBP) {
-    if (st->done) return;
-    const int cur = st->cur;
+    const int cur = bp_begin(st, cv.sweep);
+    if (cur < 0) return;
     min = cur ? msgB : msgA;
     mout = cur ? msgA : msgB;
   }
@@ -340,8 +340,7 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
       dden = fmax(dden, __shfl_xor(dden, o, 64));
     }
     if (l == 0) {
-      atomic_max_nonneg(&st->num_bits, sqrt(dnum));
-      atomic_max_nonneg(&st->den_bits, sqrt(dden));
+      bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
     }
   }
 }
