@@ -165,11 +165,13 @@ struct BpStatus {
 __device__ inline int bp_begin(BpStatus* st, int sweep) {
   if (sweep < 0) return st->done ? -1 : st->cur;
   if (sweep > 0) {
-    if (st->marker) return -1;
+    // blocks of one launch may set the marker while others read it; either
+    // outcome gives the same decision, since both follow from the same slot
+    if (__hip_atomic_load(&st->marker, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return -1;
     const double num = __longlong_as_double((long long)st->slot[sweep - 1][0]);
     const double den = __longlong_as_double((long long)st->slot[sweep - 1][1]);
     if (num / den < st->bp_eps) {
-      if (threadIdx.x == 0) st->marker = 1;
+      if (threadIdx.x == 0) __hip_atomic_store(&st->marker, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return -1;
     }
   }
