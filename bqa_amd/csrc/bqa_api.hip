@@ -159,7 +159,8 @@ struct bqa_job {
   int lstride = 1;
   int pred_sweeps = 8;
   std::vector<ClassDev> cls;
-  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws, conv, conv_msgs, zz;
+  DevBuf msgs[2], lmbd, ext, canon, lmax, status, rho, bloch, edges, epart, gws, cws, conv, conv_msgs, zz, plist;
+  std::vector<int32_t> node_cls, node_pos;  // qubit id -> (degree class, position in the class)
   bool zz_valid = false;  // zz holds the gates of the current simple update
   BpStatus* st_host = nullptr;
   double* lmax_host = nullptr;
@@ -400,6 +401,19 @@ __global__ void k_energy_final(const double* part, int nb, double* out) {
     double s = 0.0;
     for (int b = 0; b < nb; ++b) s += part[b];
     *out = s;
+  }
+}
+
+// Projective measurement of listed qubits of one class (backends.py:729-734):
+// entry x of `list` is (position in the class, outcome); the physical component
+// 1 - outcome of that node's tensor (half = chi^d elements) is set to zero.
+template <typename R>
+__global__ void k_project(cplx<R>* T, const int2* __restrict__ list, int count, long half) {
+  for (int x = blockIdx.y; x < count; x += gridDim.y) {
+    const int2 e = list[x];
+    cplx<R>* dst = T + (2 * (long)e.x + (1 - e.y)) * half;
+    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < half; k += (long)gridDim.x * blockDim.x)
+      dst[k] = mk<R>(0, 0);
   }
 }
 
@@ -773,6 +787,39 @@ void density_impl(bqa_job* j, double* rho_out, double* bloch_out) {
   HIPX(hipStreamSynchronize(j->stream));
 }
 
+// Project the listed qubits (ids[k] on outcomes[k]); grouped by class, one launch per class.
+template <typename R>
+void project_impl(bqa_job* j, int count, const int32_t* ids, const int32_t* outcomes) {
+  std::vector<std::vector<int2>> per(j->cls.size());
+  for (int k = 0; k < count; ++k) {
+    require(ids[k] >= 0 && ids[k] < j->N, "qubit id out of range");
+    require(outcomes[k] == 0 || outcomes[k] == 1, "outcome must be 0 or 1");
+    per[j->node_cls[ids[k]]].push_back(make_int2(j->node_pos[ids[k]], outcomes[k]));
+  }
+  size_t off = 0;
+  for (auto& v : per) off += v.size();
+  if (off == 0) return;
+  j->plist.ensure(off * sizeof(int2), j->dev_bytes, j->stream);
+  std::vector<int2> flat;
+  flat.reserve(off);
+  for (auto& v : per) flat.insert(flat.end(), v.begin(), v.end());
+  HIPX(hipMemcpyAsync(j->plist.p, flat.data(), off * sizeof(int2), hipMemcpyHostToDevice, j->stream));
+  off = 0;
+  for (size_t c = 0; c < per.size(); ++c) {
+    const int cnt = (int)per[c].size();
+    if (cnt == 0) continue;
+    ClassDev& cl = j->cls[c];
+    const long half = ipow_l(j->chi, cl.d);
+    const dim3 grid((unsigned)std::max<long>(1, std::min<long>((half + 255) / 256, 64)),
+                    (unsigned)std::min(cnt, 4096));
+    hipLaunchKernelGGL((k_project<R>), grid, dim3(256), 0, j->stream, reinterpret_cast<cplx<R>*>(cl.T[cl.tcur].p),
+                       reinterpret_cast<const int2*>(j->plist.p) + off, cnt, half);
+    LAUNCH_CHECK();
+    off += cnt;
+  }
+  HIPX(hipStreamSynchronize(j->stream));  // the host list buffer goes out of scope
+}
+
 template <typename R>
 void init_state_impl(bqa_job* j) {
   j->chi = 1;
@@ -827,14 +874,49 @@ void dispatch(bqa_job* j, F&& f) {
     f(double());
 }
 
-size_t estimate(int M, const std::vector<int>& deg, const std::vector<int>& size, int chimax, int prec) {
-  const size_t cs = prec == BQA_PREC_SINGLE ? 8 : 16;
+// Device bytes a job of this shape can hold at its peak: every buffer the
+// phases grow, at the largest bond dimension they reach.  Buffers only grow, so
+// the peak is the sum of each buffer's largest size over chi = 1..chimax.
+size_t estimate(int N, int M, const std::vector<int>& deg, const std::vector<int>& size, int chimax, int prec) {
+  const double cs = prec == BQA_PREC_SINGLE ? 8.0 : 16.0;
+  const double E = M / 2.0;
+  const int n2 = 2 * chimax, ccols = std::min(n2, chimax);
   double b = 0.0;
-  for (size_t c = 0; c < deg.size(); ++c) b += 2.0 * size[c] * 2.0 * ipow_d(chimax, deg[c]) * cs;
-  b += 2.0 * M * (double)chimax * chimax * cs;                  // messages
-  b += (double)M * 4.0 * chimax * chimax * 16.0;                // extended messages (FP64)
-  b += (double)M * 2.0 * chimax * chimax * sizeof(zd);          // canonicalisers
-  b += (double)(M / 2) * 2.0 * chimax * sizeof(double);         // lambdas
+  int dmax = 0;
+  long nmax = 0;
+  for (size_t c = 0; c < deg.size(); ++c) {
+    b += 2.0 * size[c] * 2.0 * ipow_d(chimax, deg[c]) * cs;                    // two tensor buffers
+    b += 4.0 * size[c] * (1 + 3.0 * deg[c]) + 8.0 * size[c] * (1 + deg[c]) + 64;  // layout arrays
+    dmax = std::max(dmax, deg[c]);
+    nmax = std::max<long>(nmax, size[c]);
+  }
+  b += 2.0 * M * (double)chimax * chimax * cs;                     // two message buffers
+  b += (double)M * n2 * n2 * 16.0;                                 // extended messages (FP64)
+  b += (double)M * n2 * ccols * sizeof(zd);                        // canonicalisers
+  b += E * n2 * sizeof(double);                                    // lambdas
+  b += E * 2.0 * 16.0;                                             // ZZ-split gates
+  // canonicaliser workspace: per edge for the chi <= 8 kernels (largest at chi = min(chimax, 8)),
+  // per workgroup for the generic kernel once its workspace leaves LDS
+  const int cr = std::min(chimax, 8);
+  const int nr = 2 * cr;
+  double cws = E * (10.0 * nr * nr + 5.0 * nr) * sizeof(double);
+  if (chimax > 8) {
+    const size_t wsb = canon_ws_bytes(n2);
+    if (wsb > (size_t)kMaxLds) cws = std::max(cws, std::min(E, 2048.0) * (double)wsb);
+  }
+  b += cws;
+  // generic node kernels' global workspace (3 node buffers per workgroup, at most 1024 workgroups) once
+  // their buffers leave LDS; FP64 elements for the extended-message and local kernels
+  const double smax = 2.0 * ipow_d(chimax, dmax);
+  if (16.0 * 3.0 * smax > (double)kMaxLds) b += 1024.0 * 3.0 * smax * 16.0;
+  // FP32 jobs whose extended messages run the generic kernel widen tensors and messages to FP64
+  if (prec == BQA_PREC_SINGLE && (chimax > 8 || dmax > 3)) {
+    b += (double)nmax * smax * 16.0;
+    b += (double)M * chimax * chimax * 16.0;
+  }
+  b += (double)N * 11.0 * sizeof(double);                          // densities and Bloch vectors
+  b += E * (2.0 * 4.0 + 8.0) + (double)N * 8.0 + 64;               // forward edges and fields
+  b += sizeof(BpStatus) + 128 * sizeof(double) + 1024 * sizeof(double) + 64;
   return (size_t)b;
 }
 
@@ -868,7 +950,10 @@ int bqa_job_estimate_bytes(int32_t edges_number, int32_t classes, const int32_t*
   return guarded([&] {
     require(bytes && class_degree && class_size && classes >= 0, "null argument");
     std::vector<int> deg(class_degree, class_degree + classes), sz(class_size, class_size + classes);
-    *bytes = estimate(edges_number, deg, sz, max_bond_dim, precision);
+    require(edges_number >= 0 && max_bond_dim >= 1 && max_bond_dim <= 32, "bad shape");
+    long n = 0;
+    for (int c = 0; c < classes; ++c) n += sz[c];
+    *bytes = estimate((int)n, edges_number, deg, sz, max_bond_dim, precision);
   });
 }
 
@@ -932,9 +1017,16 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
       cl.lp = cl.out + nb;
       cl.h = reinterpret_cast<double*>(base + ((ib + 15) / 16) * 16);
       cl.J = cl.h + cl.n;
+      if (c == 0) {
+        j->node_cls.assign(nodes_number, -1);
+        j->node_pos.assign(nodes_number, -1);
+      }
       for (long x = 0; x < cl.n; ++x) {
         const int id = node_ids[off_n + x];
         require(id >= 0 && id < nodes_number, "node id out of range");
+        require(j->node_cls[id] < 0, "node id listed twice");
+        j->node_cls[id] = c;
+        j->node_pos[id] = (int32_t)x;
         h_by_id[id] = node_ampls[off_n + x];
       }
       for (size_t x = 0; x < nb; ++x) {
@@ -994,7 +1086,8 @@ int bqa_job_destroy(bqa_job* j) {
       c.T[1].release(j->dev_bytes);
     }
     DevBuf* bufs[] = {&j->msgs[0], &j->msgs[1], &j->lmbd, &j->ext, &j->canon, &j->lmax, &j->status,
-                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws, &j->conv, &j->conv_msgs, &j->zz};
+                      &j->rho, &j->bloch, &j->edges, &j->epart, &j->gws, &j->cws, &j->conv, &j->conv_msgs, &j->zz,
+                      &j->plist};
     for (DevBuf* b : bufs) b->release(j->dev_bytes);
     for (auto& pr : j->ev_pending) {
       (void)hipEventDestroy(pr.first);
@@ -1117,6 +1210,13 @@ int bqa_energy(bqa_job* j, double* energy, int8_t* spins_out) {
       for (int v = 0; v < j->N; ++v) spins_out[v] = b[(size_t)v * 3 + 2] > 0.0 ? 1 : -1;
     }
     HIPX(hipStreamSynchronize(j->stream));
+  });
+}
+
+int bqa_project(bqa_job* j, int32_t count, const int32_t* qubit_ids, const int32_t* outcomes) {
+  return guarded([&] {
+    require(j && count >= 0 && (count == 0 || (qubit_ids && outcomes)), "null argument");
+    dispatch(j, [&](auto r) { project_impl<decltype(r)>(j, count, qubit_ids, outcomes); });
   });
 }
 

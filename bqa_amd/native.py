@@ -43,6 +43,7 @@ SIGNATURES = {
                               _i32p, _i32p, _f64p]),
     "bqa_density": (c_int, [c_void_p, _f64p, _f64p]),
     "bqa_energy": (c_int, [c_void_p, _f64p, POINTER(c_int8)]),
+    "bqa_project": (c_int, [c_void_p, c_int32, _i32p, _i32p]),
     "bqa_bond_dim": (c_int, [c_void_p, _i32p]),
     "bqa_get_tensors": (c_int, [c_void_p, c_int32, _f64p]),
     "bqa_set_tensors": (c_int, [c_void_p, c_int32, c_int32, _f64p]),
@@ -61,7 +62,10 @@ def _load() -> ctypes.CDLL:
         raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
-        fn = getattr(lib, name)
+        try:
+            fn = getattr(lib, name)
+        except AttributeError as e:     # a library built from older sources
+            raise ImportError(f"{LIB_PATH} does not export {name}: rebuild it") from e
         fn.restype = res
         fn.argtypes = args
     return lib
@@ -133,22 +137,37 @@ class Job:
     """One annealing instance resident on the GPU (a ``bqa_job`` handle)."""
 
     def __init__(self, ctx, precision: str = "single", max_bond_dim: int | None = None):
+        f = flatten_layouts(ctx)
+        f.update(lhs=_i32(ctx.edges.lhs), rhs=_i32(ctx.edges.rhs), jv=_f64(ctx.edges.couplings))
+        chi_max = int(max_bond_dim if max_bond_dim is not None else ctx.max_bond_dim)
+        self._create(f, ctx.nodes_number, ctx.edges_number, chi_max, precision)
+
+    @classmethod
+    def from_arrays(cls, arrays: dict, nodes_number: int, edges_number: int, max_bond_dim: int,
+                    precision: str = "single") -> "Job":
+        """A job from flattened layout arrays (the keys of ``flatten_layouts`` plus the forward edges
+        ``lhs``, ``rhs``, ``jv``)."""
+        self = cls.__new__(cls)
+        self._create(arrays, nodes_number, edges_number, int(max_bond_dim), precision)
+        return self
+
+    def _create(self, f: dict, nodes_number: int, edges_number: int, chi_max: int, precision: str) -> None:
+        self.handle = None
         if precision not in PREC:
             raise ValueError(f"unknown precision {precision}")
         self.precision = precision
-        self.nodes_number = ctx.nodes_number
-        self.edges_number = ctx.edges_number
-        self.classes = [(l.degree, l.size) for l in ctx.degree_to_layout.values()]
-        f = flatten_layouts(ctx)
+        self.nodes_number = int(nodes_number)
+        self.edges_number = int(edges_number)
+        f = {k: (_i32(v) if k in ("degree", "size", "ids", "inp", "outp", "lp", "lhs", "rhs") else _f64(v))
+             for k, v in f.items()}
+        self.classes = list(zip(f["degree"].tolist(), f["size"].tolist()))
         self._keep = f
-        lhs, rhs, jv = _i32(ctx.edges.lhs), _i32(ctx.edges.rhs), _f64(ctx.edges.couplings)
         h = c_void_p()
-        chi_max = int(max_bond_dim if max_bond_dim is not None else ctx.max_bond_dim)
         check(LIB.bqa_job_create(
-            ctypes.byref(h), ctx.nodes_number, ctx.edges_number, len(self.classes),
+            ctypes.byref(h), self.nodes_number, self.edges_number, len(self.classes),
             _p(f["degree"], c_int32), _p(f["size"], c_int32), _p(f["ids"], c_int32), _p(f["inp"], c_int32),
             _p(f["outp"], c_int32), _p(f["lp"], c_int32), _p(f["h"], c_double), _p(f["j"], c_double),
-            _p(lhs, c_int32), _p(rhs, c_int32), _p(jv, c_double), chi_max, PREC[precision]))
+            _p(f["lhs"], c_int32), _p(f["rhs"], c_int32), _p(f["jv"], c_double), chi_max, PREC[precision]))
         self.handle = h
 
     # lifecycle
@@ -211,6 +230,13 @@ class Job:
         spins = np.empty(self.nodes_number, dtype=np.int8)
         check(LIB.bqa_energy(self.handle, ctypes.byref(e), spins.ctypes.data_as(POINTER(c_int8))))
         return e.value, spins
+
+    def project(self, qubit_ids, outcomes) -> None:
+        """Project qubits on outcomes 0 (|0>) / 1 (|1>) (``bqa_project``)."""
+        ids, out = _i32(qubit_ids), _i32(outcomes)
+        if ids.shape != out.shape:
+            raise ValueError("qubit_ids and outcomes differ in length")
+        check(LIB.bqa_project(self.handle, int(ids.size), _p(ids, c_int32), _p(out, c_int32)))
 
     @property
     def bond_dim(self) -> int:

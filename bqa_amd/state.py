@@ -35,9 +35,13 @@ class State:
 
 
 def _initialize_state(context) -> State:
-    """|-> product state, bond dimension 1 (src/bqa/state.py:67-74)."""
-    cls = BACKEND_STR_TO_BACKEND[context.backend]
-    backend = cls(context, context.precision)
+    """|-> product state, bond dimension 1 (src/bqa/state.py:67-74).
+
+    ``context.backend`` is a registered name (this package's Context) or the backend class itself (the
+    reference's Context, config_canonicalization.py:211)."""
+    b = context.backend
+    cls = b if isinstance(b, type) else BACKEND_STR_TO_BACKEND[b]
+    backend = cls(context)
     backend.init_state()
     return State(backend=backend, rng=default_rng(context.seed))
 
@@ -113,6 +117,74 @@ def get_bloch_vectors(context, state: State) -> np.ndarray:
 
 
 def measure(context, state: State) -> list:
-    """Sequential projective sampling (src/bqa/state.py:250-312) is not provided by this backend."""
-    raise NotImplementedError("the `measure` instruction is not supported by the mi355x backend; "
-                              "use `get_bloch_vectors` and round the z components")
+    """Sequential projective sampling (src/bqa/state.py:250-312).
+
+    Until every qubit is measured: read the densities on the GPU; among the unmeasured qubits take the
+    one whose ground probability p = rho[0, 0] has the largest |2p - 1| (the lowest id on ties, as
+    ``max`` over the reference's id-ordered dict); draw u = rng.uniform(0, 1) from the state's seeded
+    generator (state.py:70) and take outcome 0 iff p > u; project it (``bqa_project``) and run BP; then
+    from one fresh set of densities project every unmeasured qubit with p > threshold on 0 and every one
+    with p < 1 - threshold on 1; run BP again.  Returns +1 (outcome 0) / -1 (outcome 1) per qubit id.
+    The draws, the choices and the thresholds run on the host in FP64, exactly as the reference's.
+    """
+    n = context.nodes_number
+    thr = context.measurement_threshold
+    rng = state.rng
+    outcomes = np.zeros(n, dtype=np.int64)          # +1 / -1 once measured
+    measured = np.zeros(n, dtype=bool)
+
+    def ground_probs() -> np.ndarray:
+        rho, _ = state.backend.density()
+        return rho[:, 0, 0].real.astype(np.float64)
+
+    def register(ids, outs) -> None:
+        ids = np.asarray(ids, dtype=np.int64)
+        outs = np.asarray(outs, dtype=np.int64)
+        if ids.size:
+            outcomes[ids] = 1 - 2 * outs
+            measured[ids] = True
+            state.backend.project(ids, outs)
+
+    def bp() -> None:
+        sweeps, converged, dist = state.backend.run_bp(context.max_bp_iters_number, context.bp_eps, context.damping)
+        _log_bp(context, sweeps, converged, dist)
+
+    log.debug("Measurement outcomes sampling started")
+    while not measured.all():
+        p = ground_probs()
+        open_ids = np.flatnonzero(~measured)
+        score = np.abs(2.0 * p[open_ids] - 1.0)
+        v = int(open_ids[int(np.argmax(score))])      # first maximum in id order
+        u = rng.uniform(0.0, 1.0)
+        register([v], [0 if p[v] > u else 1])
+        bp()
+        p = ground_probs()
+        open_ids = np.flatnonzero(~measured)
+        up = open_ids[p[open_ids] > thr]
+        down = open_ids[p[open_ids] < 1.0 - thr]
+        register(up, np.zeros(up.size, dtype=np.int64))
+        register(down, np.ones(down.size, dtype=np.int64))
+        bp()
+    log.info("Measurement outcomes sampling completed")
+    return [int(x) for x in outcomes]
+
+
+def run_instructions(context) -> list:
+    """The instruction interpreter of src/bqa/core.py:13-35 over a built context of either kind."""
+    n_instr = len(context.instructions)
+    state = _initialize_state(context)
+    results = []
+    try:
+        for number, instruction in enumerate(context.instructions):
+            log.info(f"Instruction number {number} / {n_instr} started")
+            if isinstance(instruction, dict):
+                run_layer(context, instruction["xtime"], instruction["ztime"], state)
+            elif instruction == "measure":
+                results.append(["measurement_outcomes", measure(context, state)])
+            elif instruction == "get_bloch_vectors":
+                results.append(["bloch_vectors", get_bloch_vectors(context, state).tolist()])
+            else:
+                raise ValueError(f"Unknown instruction {instruction}")
+    finally:
+        state.backend.close()
+    return results

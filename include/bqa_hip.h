@@ -131,6 +131,18 @@ int bqa_density(bqa_job* job, double* rho_out, double* bloch_out);
  * bqa_density call: sum_i h_i s_i + sum_(ij) J_ij s_i s_j; spins_out (n int8) may be NULL. */
 int bqa_energy(bqa_job* job, double* energy, int8_t* spins_out);
 
+/*
+ * Projective measurement of qubits (replaces Tensor.measure,
+ * src/bqa/backends.py:309-310,729-734, as called by state.py:281-285):
+ * for each k, the physical component 1 - outcomes[k] (outcome 0 = |0>,
+ * 1 = |1>) of qubit qubit_ids[k]'s tensor is set to zero.  The reference also
+ * divides the whole degree class by its Frobenius norm; every quantity read
+ * afterwards (trace-normalised messages, densities, the per-node gauge
+ * normalisation) is invariant to that scale, so it is not applied here.
+ * The sampling loop itself (state.py:250-312) runs on the host.
+ */
+int bqa_project(bqa_job* job, int32_t count, const int32_t* qubit_ids, const int32_t* outcomes);
+
 /* Current bond dimension. */
 int bqa_bond_dim(const bqa_job* job, int32_t* chi);
 

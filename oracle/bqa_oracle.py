@@ -541,8 +541,62 @@ def bloch_vectors(rho: np.ndarray) -> np.ndarray:
     return np.stack([x, y, z], axis=1)
 
 
+def project(ctx: Context, st: State, node: int, outcome: int) -> None:
+    """Project qubit ``node`` on outcome 0 (|0>) or 1 (|1>): zero the other physical component of
+    its tensor, then divide the whole degree-class batch by its Frobenius norm
+    (src/bqa/backends.py:309-310,729-734)."""
+    for d, lay in ctx.classes.items():
+        hit = np.nonzero(lay.node_ids == node)[0]
+        if hit.size:
+            t = st.tensors[d]
+            t[hit[0], 1 - outcome] = 0.0
+            t /= np.linalg.norm(t)
+            return
+    raise KeyError(node)
+
+
+def measure(ctx: Context, st: State) -> list:
+    """Sequential projective sampling (src/bqa/state.py:250-312).
+
+    Until every qubit is measured: among the unmeasured qubits take the one whose ground
+    probability p = rho[0, 0] has the largest |2p - 1| (the first such id on ties, as
+    Python's ``max`` over the id-ordered dict); draw u = rng.uniform(0, 1) from the state's
+    generator and take outcome 0 iff p > u; project and run BP; then, from one fresh set of
+    densities, project every unmeasured qubit with p > threshold on 0 and every one with
+    p < 1 - threshold on 1; run BP again.  Returns +1 (outcome 0) / -1 (outcome 1) per id.
+    """
+    outcomes = {}
+    thr = ctx.measurement_threshold
+    n_layers = len(st.bp_iters)       # the sweep counts of this sampling are not Trotter steps
+
+    def ground_probs():
+        rho = get_density_matrices(ctx, st)
+        return {v: float(rho[v, 0, 0].real) for v in range(ctx.nodes_number) if v not in outcomes}
+
+    def register(v, outcome):
+        outcomes[v] = 1 - 2 * outcome
+        project(ctx, st, v, outcome)
+
+    while len(outcomes) < ctx.nodes_number:
+        probs = ground_probs()
+        v, p = max(probs.items(), key=lambda kv: abs(2 * kv[1] - 1))
+        u = st.rng.uniform(0.0, 1.0)
+        register(v, 0 if p > u else 1)
+        run_bp(ctx, st)
+        probs = ground_probs()
+        for w, q in list(probs.items()):
+            if q > thr:
+                register(w, 0)
+        for w, q in probs.items():
+            if q < 1.0 - thr:
+                register(w, 1)
+        run_bp(ctx, st)
+    del st.bp_iters[n_layers:]
+    return [outcomes[v] for v in range(len(outcomes))]
+
+
 def run_qa(config: dict, state_out: Optional[list] = None) -> list:
-    """src/bqa/core.py:13-35 without the measurement instruction."""
+    """src/bqa/core.py:13-35."""
     ctx = make_context(config)
     st = init_state(ctx)
     results = []
@@ -552,7 +606,7 @@ def run_qa(config: dict, state_out: Optional[list] = None) -> list:
         elif ins == "get_bloch_vectors":
             results.append(["bloch_vectors", bloch_vectors(get_density_matrices(ctx, st)).tolist()])
         elif ins == "measure":
-            raise NotImplementedError("sequential measurement is outside the oracle")
+            results.append(["measurement_outcomes", measure(ctx, st)])
         else:
             raise ValueError(f"Unknown instruction {ins}")
     if state_out is not None:

@@ -28,7 +28,8 @@ def load_golden(name):
     edges = {(int(a), int(b)): float(c) for (a, b), c in zip(z["edges"], z["couplings"])}
     nodes = {v: float(h) for v, h in enumerate(z["fields"])}
     cfg = dict(params, nodes=nodes, edges=edges)
-    return cfg, {k: z[k] for k in ("bloch", "chis", "sweeps", "lmbds")}
+    keys = ("bloch", "chis", "sweeps", "lmbds", "outcomes")
+    return cfg, {k: z[k] for k in keys if k in z.files}
 
 
 @pytest.fixture(scope="session")

@@ -1,7 +1,8 @@
 """Parity of the HIP path (through the C ABI) with the FP64 oracle and the reference's golden vectors.
 
 Tolerances: FP64 GPU path 1e-8 on Bloch vectors (same trajectory, same bond
-dimensions, same BP sweep counts); FP32 path 1e-4 on Bloch vectors.  Phase
+dimensions, same BP sweep counts); FP32 path 1e-5 on Bloch vectors (the
+tolerance stated for FP32 paths against the FP64 oracle).  Phase
 checks compare gauge-fixed quantities only (messages for given tensors,
 tensors after local gates and gauge, densities, lambdas).
 """
@@ -18,7 +19,7 @@ from treeutil import SMALL_TREE_CONFIG, branch_message, per_node, random_tree_te
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"double": 1e-8, "single": 1e-4}
+TOL = {"double": 1e-8, "single": 1e-5}
 
 
 def run_product(cfg, precision, fused=True):
@@ -37,18 +38,34 @@ def run_product(cfg, precision, fused=True):
         st.backend.close()
 
 
+def fp64_tolerance(cfg) -> float:
+    """1e-8 on Bloch vectors; 1e-7 where pinv_eps is 1e-12 (the chi = 16 / 32 heavy-hex fixtures): the
+    canonicalisers multiply by S^-1/2 on singular values down to 1e-12, which amplifies the FP64 rounding
+    difference between the Jacobi SVD here and the reference's LAPACK SVD by up to 1e6."""
+    return TOL["double"] if cfg["pinv_eps"] >= 1e-9 else 1e-7
+
+
 @pytest.mark.parametrize("name", golden_names())
 def test_golden_fp64(name):
     cfg, want = load_golden(name)
     print(f"[gpu] golden fp64 {name}", flush=True)
     bloch, st, lm = run_product(cfg, "double")
-    assert np.abs(bloch - want["bloch"]).max() < TOL["double"]
+    err = np.abs(bloch - want["bloch"]).max()
+    print(f"[gpu]   fp64 max |bloch error| = {err:.3e}", flush=True)
     assert st.chis == want["chis"].tolist()
     assert st.bp_sweeps == want["sweeps"].tolist()
-    assert np.abs(lm - want["lmbds"]).max() < 1e-8
+    assert err < fp64_tolerance(cfg)
+    assert np.abs(lm - want["lmbds"]).max() < fp64_tolerance(cfg)
 
 
-@pytest.mark.parametrize("name", golden_names())
+def fp32_golden_names():
+    """Fixtures whose pinv_eps is above FP32 rounding: with pinv_eps = 1e-12 (the chi = 16 and 32
+    heavy-hex fixtures) the rank cut of FP32 messages falls inside their rounding noise, so the
+    bond-dimension sequence is not defined at FP32 precision."""
+    return [n for n in golden_names() if load_golden(n)[0]["pinv_eps"] >= 1e-7]
+
+
+@pytest.mark.parametrize("name", fp32_golden_names())
 def test_golden_fp32(name):
     cfg, want = load_golden(name)
     print(f"[gpu] golden fp32 {name}", flush=True)
@@ -290,9 +307,8 @@ FAST_CASES = {
 
 @pytest.mark.parametrize("name", sorted(FAST_CASES))
 def test_fast_kernels_end_to_end(name):
-    """Specialised kernels end to end: FP64 equals the oracle; FP32 stays within 1e-3 of it
-    (FP32 rounding of the extended messages is amplified by S^-1/2 on Schmidt values near
-    pinv_eps, so sensitive instances drift further than the golden fixtures)."""
+    """Specialised kernels end to end: FP64 equals the oracle to 1e-8; FP32 stays within the
+    FP32 tolerance of 1e-5 of it."""
     nodes, edges = FAST_CASES[name]()
     chi = 4 if name.endswith("chi4") else 8
     cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "bp_eps": 1e-9, "max_bp_iter_number": 60,
@@ -303,12 +319,13 @@ def test_fast_kernels_end_to_end(name):
     fast64, _ = _run_bloch(cfg, "double", generic=False)
     print(f"[gpu]   chis {st.chis[-3:]} |fp32-oracle| {np.abs(fast32 - want).max():.2e} "
           f"|fp64-oracle| {np.abs(fast64 - want).max():.2e}", flush=True)
-    assert np.abs(fast32 - want).max() < 1e-3
-    assert np.abs(fast64 - want).max() < 1e-8
+    assert np.abs(fast32 - want).max() < TOL["single"]
+    assert np.abs(fast64 - want).max() < TOL["double"]
 
 
 def _job_from_oracle(cfg, ctx_o, st_o, precision, generic):
     import os
+    old = os.environ.get("BQA_GENERIC_KERNELS")
     os.environ["BQA_GENERIC_KERNELS"] = "1" if generic else "0"
     try:
         job = Job(config_to_context(dict(cfg, backend="mi355x")), precision)
@@ -316,7 +333,10 @@ def _job_from_oracle(cfg, ctx_o, st_o, precision, generic):
         _upload(job, ctx_o, st_o)
         return job
     finally:
-        os.environ.pop("BQA_GENERIC_KERNELS", None)
+        if old is None:
+            os.environ.pop("BQA_GENERIC_KERNELS", None)
+        else:
+            os.environ["BQA_GENERIC_KERNELS"] = old
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
@@ -425,3 +445,30 @@ def test_small_kernels_match_generic(name, precision):
     finally:
         for job in jobs:
             job.close()
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("chi", [4, 8])
+def test_estimate_covers_device_bytes(chi, precision):
+    """bqa_job_estimate_bytes (the serving admission figure) bounds what a job holds at its peak."""
+    from bqa_amd.native import estimate_bytes
+    nodes, edges, _ = instances.regular_qubo(2000, 3, seed=7, qmax=10)
+    cfg = {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "schedule": instances.anneal_schedule(20.0, 40),
+           "backend": "mi355x", "precision": precision}
+    ctx = config_to_context(cfg)
+    est = estimate_bytes(ctx, chi, precision)
+    job = Job(ctx, precision)
+    try:
+        job.init_state()
+        peak = 0
+        for ins in [i for i in ctx.instructions if isinstance(i, dict)]:
+            out = job.run_layer(ins["xtime"], ins["ztime"], 1e-6, 75, 1e-6, 0.0)
+            peak = max(peak, job.device_bytes())
+        job.density()
+        peak = max(peak, job.device_bytes())
+        print(f"[gpu] estimate chi={chi} {precision}: bond dim {out['bond_dim']}, held {peak}, "
+              f"estimate {est}", flush=True)
+        assert out["bond_dim"] == chi
+        assert peak <= est
+    finally:
+        job.close()

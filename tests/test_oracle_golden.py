@@ -24,6 +24,9 @@ def test_oracle_matches_reference_fixture(name):
     lm = np.sort(st.lmbds.real, axis=1)[:, ::-1]
     assert lm.shape == want["lmbds"].shape
     assert np.abs(lm - want["lmbds"]).max() < 1e-10
+    if "outcomes" in want:
+        got = [r[1] for r in res if r[0] == "measurement_outcomes"]
+        assert np.array_equal(np.array(got), want["outcomes"])
 
 
 def test_fixtures_exercise_the_exit_rules():
@@ -34,3 +37,11 @@ def test_fixtures_exercise_the_exit_rules():
     assert (nc["sweeps"] == 3).any()          # hit the limit of 3 sweeps
     cfg, _ = load_golden("rr100_damped")
     assert cfg["damping"] > 0
+
+
+def test_measurement_fixtures_sample_both_outcomes():
+    """The sampling fixtures draw both outcomes, so a flipped comparison would fail them."""
+    for name in ("rr24_measure_chi4", "grid4x5_measure_chi8"):
+        _, want = load_golden(name)
+        out = want["outcomes"]
+        assert out.shape[0] == 1 and set(np.unique(out)) == {-1, 1}
