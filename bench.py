@@ -305,7 +305,7 @@ def main():
         raise RuntimeError("no BP sweep kernel was timed at the target bond dimension")
     if s["chi_timed"] != [args.chi]:
         raise RuntimeError(f"timed steps ran at bond dimensions {s['chi_timed']}, not {args.chi}")
-    kname = {"double": "k_mfma8_bp64<3,2> (FP64 MFMA)", "single": "k_mfma8<3,kModeBP,1> (FP32 MFMA)"}
+    kname = {"double": "k_mfma8_bp64_3p<4> (FP64 MFMA, six contractions per node)", "single": "k_mfma8<3,kModeBP,1> (FP32 MFMA)"}
     roofline = roofline_of(s, head["ctx"], args.chi, args.precision,
                            f"BP sweep at chi={args.chi}, d=3: {kname[args.precision] if args.chi == 8 else 'per shape'}"
                            " (one launch per degree class per sweep)")

@@ -909,10 +909,12 @@ size_t estimate(int N, int M, const std::vector<int>& deg, const std::vector<int
   // their buffers leave LDS; FP64 elements for the extended-message and local kernels
   const double smax = 2.0 * ipow_d(chimax, dmax);
   if (16.0 * 3.0 * smax > (double)kMaxLds) b += 1024.0 * 3.0 * smax * 16.0;
-  // FP32 jobs whose extended messages run the generic kernel widen tensors and messages to FP64
-  if (prec == BQA_PREC_SINGLE && (chimax > 8 || dmax > 3)) {
-    b += (double)nmax * smax * 16.0;
-    b += (double)M * chimax * chimax * 16.0;
+  // FP32 jobs whose extended messages run the generic kernel widen tensors and messages to FP64: at
+  // bond dimension 1 (the first step) always, at chimax when no specialised kernel serves the shape
+  if (prec == BQA_PREC_SINGLE) {
+    const bool generic_at_max = chimax > 8 || dmax > 3;
+    b += (double)nmax * (generic_at_max ? smax : 2.0) * 16.0;
+    b += (double)M * (generic_at_max ? (double)chimax * chimax : 1.0) * 16.0;
   }
   b += (double)N * 11.0 * sizeof(double);                          // densities and Bloch vectors
   b += E * (2.0 * 4.0 + 8.0) + (double)N * 8.0 + 64;               // forward edges and fields

@@ -139,21 +139,16 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
   return true;
 }
 
-// FP64, chi = 8, degree 2 or 3: FP64 MFMA (k_mfma8_bp64); degree 3 splits each
-// node over BQA_BP64_WAVES waves.
-#ifndef BQA_BP64_WAVES
-#define BQA_BP64_WAVES 2
-#endif
-
+// FP64, chi = 8: FP64 MFMA; degree 3 in six contractions per node over four
+// waves (k_mfma8_bp64_3p), degree 2 one wave per node (k_mfma8_bp64).
 template <>
 inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
                                    BpStatus* st, hipStream_t s) {
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeBP, double, double>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
-  constexpr int W = BQA_BP64_WAVES;
   if (cv.d == 3)
-    hipLaunchKernelGGL((k_mfma8_bp64<3, W>), dim3(node_grid<k_mfma8_bp64<3, W>, 64 * W>(cv.n)), dim3(64 * W), 0, s,
-                       cv, T, mA, mB, st);
+    hipLaunchKernelGGL((k_mfma8_bp64_3p<4>), dim3(node_grid<k_mfma8_bp64_3p<4>, 256>(cv.n)), dim3(256), 0, s, cv, T,
+                       mA, mB, st);
   else
     hipLaunchKernelGGL((k_mfma8_bp64<2, 1>), dim3(node_grid<k_mfma8_bp64<2, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
                        mA, mB, st);

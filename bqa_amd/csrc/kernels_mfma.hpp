@@ -511,6 +511,29 @@ __device__ __forceinline__ void msg_operand8d(const cplx<R>* __restrict__ m, dou
   }
 }
 
+// msg_operand8d in two halves, so the loads can be issued long before their
+// values are turned into MFMA operands: the raw message entries this lane needs,
+// then the real-embedded operand from them.
+__device__ __forceinline__ void msg_raw8d(const cplx<double>* __restrict__ m, double2 (&raw)[4]) {
+  const int l = threadIdx.x & 63, a = l & 7, kq = l >> 4;
+  const double2* m2 = reinterpret_cast<const double2*>(m);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) raw[s] = m2[a * 8 + ((4 * s + kq) & 7)];
+}
+
+__device__ __forceinline__ void msg_operand_from_raw8d(const double2 (&raw)[4], double (&bm)[4]) {
+  const int l = threadIdx.x & 63, cn = (l & 15) >> 3, kq = l >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int ck = (4 * s + kq) >> 3;
+    // the empty asm keeps both components in registers: a lane-dependent choice
+    // between them would otherwise become an indexed access to a stack copy
+    double re = raw[s].x, im = raw[s].y;
+    asm volatile("" : "+v"(re), "+v"(im));
+    bm[s] = ck == 0 ? (cn == 0 ? re : im) : (cn == 0 ? -im : re);
+  }
+}
+
 // LDS double-index swizzle of the FP64 kernels' tensor buffers (natural index
 // 2 x element + component): bit j (j < 5) of a stored index is flipped by the
 // parity of (index & swz64_mask<D>(j)).  Linear over GF(2) like swz, so an
@@ -789,11 +812,9 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_ext64(ClassView cv, const cplx
 }
 
 // ---------------------------------------------------------------------------
-// BP sweep of an FP64 job, chi = 8, degree 2 or 3: the FP32 kernel's schedule
-// (one work buffer, mode products in place along their bond, T x2 m2 computed
-// twice for degree 3) on v_mfma_f64_16x16x4f64 over the natural layout of the
-// FP64 extended-message kernel, one block of W waves per node.  Each wave
-// stores its Gram partial (both p halves summed) in its own slot; the first
+// BP sweep of an FP64 job, chi = 8, degree 2: per output, one mode product and
+// one Gram on v_mfma_f64_16x16x4f64 over the swizzled FP64 layout of the
+// extended-message kernel (degree 3 runs k_mfma8_bp64_3p below).  The first
 // wave finishes the message while the others start the next mode product,
 // whose barrier keeps the partials alive until it has read them.
 template <int D, int W>
@@ -817,7 +838,7 @@ __device__ __forceinline__ void gram8_bp64(const double* T, const double* Y, dou
 template <int D, int W>
 __global__ __launch_bounds__(64 * W) void k_mfma8_bp64(ClassView cv, const cplx<double>* __restrict__ T,
                                                        cplx<double>* msgA, cplx<double>* msgB, BpStatus* st) {
-  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(D == 2, "degree 2 (degree 3: k_mfma8_bp64_3p)");
   static_assert(Mf8<D>::TILES % W == 0 && Mf8<D>::KS % W == 0, "tiles and k-steps split evenly over the waves");
   static_assert(Mf8<D>::S % (64 * W) == 0, "tensor load in whole rounds");
   __shared__ __align__(16) Mf8Bp64Smem<D, W> sm;
@@ -849,40 +870,149 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_bp64(ClassView cv, const cplx<
       }
     }
     node_sync<W>();
-    if constexpr (D == 2) {
-      mp8dw<D, W, 1, false>(sm.T, sm.W1, bm[1], bm[1], zl);   // Y0 = T x1 m1
-      node_sync<W>();
-      gram8_bp64<D, W, 0>(sm.T, sm.W1, sm.Gp, zl);
-      node_sync<W>();
-      if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
-      mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);   // Y1 = T x0 m0
-      node_sync<W>();
-      gram8_bp64<D, W, 1>(sm.T, sm.W1, sm.Gp, zl);
-      node_sync<W>();
-      if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
-    } else {
-      mp8dw<D, W, 2, false>(sm.T, sm.W1, bm[2], bm[2], zl);   // W1 = T x2 m2
-      node_sync<W>();
-      mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);  // Y0 = W1 x1 m1
-      node_sync<W>();
-      gram8_bp64<D, W, 0>(sm.T, sm.W1, sm.Gp, zl);
-      node_sync<W>();
-      if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
-      mp8dw<D, W, 2, false>(sm.T, sm.W1, bm[2], bm[2], zl);   // W1 = T x2 m2
-      node_sync<W>();
-      mp8dw<D, W, 0, false>(sm.W1, sm.W1, bm[0], bm[0], zl);  // Y1 = W1 x0 m0
-      node_sync<W>();
-      gram8_bp64<D, W, 1>(sm.T, sm.W1, sm.Gp, zl);
-      node_sync<W>();
-      if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
-      mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);   // W1 = T x0 m0
-      node_sync<W>();
-      mp8dw<D, W, 1, false>(sm.W1, sm.W1, bm[1], bm[1], zl);  // Y2 = W1 x1 m1
-      node_sync<W>();
-      gram8_bp64<D, W, 2>(sm.T, sm.W1, sm.Gp, zl);
-      node_sync<W>();
-      if (lead) finish8<W, double>(sm.Gp, pos[2], old[2], mout, dnum, dden);
+    mp8dw<D, W, 1, false>(sm.T, sm.W1, bm[1], bm[1], zl);   // Y0 = T x1 m1
+    node_sync<W>();
+    gram8_bp64<D, W, 0>(sm.T, sm.W1, sm.Gp, zl);
+    node_sync<W>();
+    if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
+    mp8dw<D, W, 0, false>(sm.T, sm.W1, bm[0], bm[0], zl);   // Y1 = T x0 m0
+    node_sync<W>();
+    gram8_bp64<D, W, 1>(sm.T, sm.W1, sm.Gp, zl);
+    node_sync<W>();
+    if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
+    // the next node's tensor load overwrites T, which the last Gram read
+    node_sync<W>();
+  }
+  if (lead) {
+    for (int o = 32; o > 0; o >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+      dden = fmax(dden, __shfl_xor(dden, o, 64));
     }
+    dnum = sqrt(dnum);  // the partial maxima were of squared magnitudes
+    dden = sqrt(dden);
+    if (l == 0) {
+      bp_end(st, cv.sweep, dnum, dden);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BP sweep of an FP64 job, chi = 8, degree 3, in six contractions per node
+// instead of the reference schedule's nine (backends.py:381-404 applies d - 1
+// messages to the ket side of each output).  The incoming messages are
+// Hermitian (every BP kernel stores the Hermitian part, the gauge writes real
+// diagonals), so a message on the bra side can be moved onto the ket side of a
+// conjugated factor: sum_a conj(T[..a..]) m[a][b] = conj(Y[..b..]) with
+// Y = T x m.  With Y_j = T x_j m_j for each bond j,
+//   out_0 = sum conj(Y_2) Y_1 over bonds 1, 2;  out_1 = sum conj(Y_2) Y_0 over bonds 0, 2;
+//   out_2 = sum conj(Y_1) Y_0 over bonds 0, 1,
+// three mode products and three Grams.  LDS holds T (then Y_2 in place), Y_0
+// and Y_1: 48 KB of tensors plus the Gram partials per node, so two blocks of
+// four waves per CU (100k qubits: 1.21 ms per sweep against 1.35 ms for the
+// nine-contraction schedule with two waves per node; 1.28 ms without the
+// register prefetch below).
+template <int W>
+struct Mf8Bp64S3Smem {
+  double T[2 * Mf8<3>::S];
+  double Y0[2 * Mf8<3>::S];
+  double Y1[2 * Mf8<3>::S];
+  double Gp[W][256];
+};
+
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_mfma8_bp64_3p(ClassView cv, const cplx<double>* __restrict__ T,
+                                                          cplx<double>* msgA, cplx<double>* msgB, BpStatus* st) {
+  constexpr int D = 3;
+  static_assert(Mf8<D>::TILES % W == 0 && Mf8<D>::KS % W == 0, "tiles and k-steps split evenly over the waves");
+  static_assert(Mf8<D>::S % (64 * W) == 0, "tensor load in whole rounds");
+  __shared__ __align__(16) Mf8Bp64S3Smem<W> sm;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool lead = W == 1 || wave == 0;
+  const cplx<double>* min = cur ? msgB : msgA;
+  cplx<double>* mout = cur ? msgA : msgB;
+  double dnum = 0.0, dden = 0.0;
+  constexpr int S = Mf8<D>::S;
+  const Lin8Lanes<D> zl = lin8_lanes<D, W>();
+  const int tsw = swz64<D>(2 * (int)threadIdx.x);
+  // Everything a node reads from HBM (its tensor, the three incoming messages as
+  // MFMA operands, the old messages at its outputs) is loaded into registers one
+  // node ahead, while the previous node computes, and the message positions two
+  // nodes ahead, so no load waits on another load issued in the same node: with
+  // three or fewer nodes per CU the latency of these loads is as long as a
+  // node's arithmetic.
+  constexpr int TPL = S / (64 * W);       // complex tensor elements per thread
+  const int G = gridDim.x;
+  // registers of the next node (nt, nm, nold, npos) and the positions of the one after it (pin, pout)
+  cplx<double> nt[TPL], nold[D];
+  double2 nm[D][4];
+  int npos[D], pin[D], pout[D];
+  if (blockIdx.x < n) {
+    const int i0 = blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) nt[k] = T[(size_t)i0 * S + threadIdx.x + 64 * W * k];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      msg_raw8d(min + (size_t)cv.in_pos[k * n + i0] * 64, nm[k]);
+      npos[k] = cv.out_pos[k * n + i0];
+      nold[k] = lead ? min[(size_t)npos[k] * 64 + l] : mk(0.0, 0.0);
+    }
+  }
+  if (blockIdx.x + G < n) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      pin[k] = cv.in_pos[k * n + blockIdx.x + G];
+      pout[k] = cv.out_pos[k * n + blockIdx.x + G];
+    }
+  }
+  for (int i = blockIdx.x; i < n; i += G) {
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) st_pair64(sm.T, pair64<D, 64 * W>(tsw, k), nt[k]);
+    double bm[D][4];
+    int pos[D];
+    cplx<double> old[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      msg_operand_from_raw8d(nm[k], bm[k]);
+      pos[k] = npos[k];
+      old[k] = nold[k];
+    }
+    const int i1 = i + G, i2 = i + 2 * G;
+    if (i1 < n) {
+#pragma unroll
+      for (int k = 0; k < TPL; ++k) nt[k] = T[(size_t)i1 * S + threadIdx.x + 64 * W * k];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        msg_raw8d(min + (size_t)pin[k] * 64, nm[k]);
+        npos[k] = pout[k];
+        nold[k] = lead ? min[(size_t)pout[k] * 64 + l] : mk(0.0, 0.0);
+      }
+    }
+    if (i2 < n) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        pin[k] = cv.in_pos[k * n + i2];
+        pout[k] = cv.out_pos[k * n + i2];
+      }
+    }
+    node_sync<W>();
+    mp8dw<D, W, 0, false>(sm.T, sm.Y0, bm[0], bm[0], zl);   // Y0 = T x0 m0
+    mp8dw<D, W, 1, false>(sm.T, sm.Y1, bm[1], bm[1], zl);   // Y1 = T x1 m1
+    node_sync<W>();
+    gram8_bp64<D, W, 2>(sm.Y1, sm.Y0, sm.Gp, zl);           // out_2: conj(Y1) Y0
+    node_sync<W>();
+    if (lead) finish8<W, double>(sm.Gp, pos[2], old[2], mout, dnum, dden);
+    mp8dw<D, W, 2, false>(sm.T, sm.T, bm[2], bm[2], zl);    // T <- Y2 = T x2 m2, in place
+    node_sync<W>();
+    gram8_bp64<D, W, 0>(sm.T, sm.Y1, sm.Gp, zl);            // out_0: conj(Y2) Y1
+    node_sync<W>();
+    if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
+    node_sync<W>();                                          // the partials are read before the next Gram
+    gram8_bp64<D, W, 1>(sm.T, sm.Y0, sm.Gp, zl);            // out_1: conj(Y2) Y0
+    node_sync<W>();
+    if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
     // the next node's tensor load overwrites T, which the last Gram read
     node_sync<W>();
   }
