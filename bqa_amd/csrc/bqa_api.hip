@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -416,6 +417,14 @@ __global__ void k_project(cplx<R>* T, const int2* __restrict__ list, int count, 
       dst[k] = mk<R>(0, 0);
   }
 }
+
+// Scratch of the allocation-free energy entry point (bqa_energy_noalloc): the
+// block partial sums of k_energy_partial.  One copy per GPU, held until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on each device (scratch_on_device).
+constexpr int kEnergyBlocks = 512;
+constexpr int32_t kNoAllocMaxItems = 1 << 30;
+__device__ __align__(256) double scratch[kEnergyBlocks];
 
 __global__ void k_copy_bytes(const uint4* __restrict__ src, uint4* __restrict__ dst, long n) {
   for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n; x += (long)gridDim.x * blockDim.x)
@@ -844,6 +853,25 @@ void init_state_impl(bqa_job* j) {
   j->pred_sweeps = 8;
 }
 
+// Device address of `scratch` on the current device, looked up once per device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+double* scratch_on_device() {
+  int dev = 0;
+  HIPX(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "no device address for the energy scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  return static_cast<double*>(g_scratch_ptr[dev]);
+}
+
 // ------------------------------------------------------- state transfer --
 
 template <typename R>
@@ -1198,7 +1226,7 @@ int bqa_energy(bqa_job* j, double* energy, int8_t* spins_out) {
     const double* jv = reinterpret_cast<const double*>(base + ((sizeof(int32_t) * 2 * E + 15) / 16) * 16);
     const double* hv = jv + E;
     double* part = reinterpret_cast<double*>(j->epart.p);
-    const int nb = 512;
+    const int nb = kEnergyBlocks;
     hipLaunchKernelGGL(k_energy_partial, dim3(nb), dim3(256), 0, j->stream,
                        reinterpret_cast<const double*>(j->bloch.p), j->N, hv, lhs, rhs, jv, E, part);
     LAUNCH_CHECK();
@@ -1219,6 +1247,27 @@ int bqa_project(bqa_job* j, int32_t count, const int32_t* qubit_ids, const int32
   return guarded([&] {
     require(j && count >= 0 && (count == 0 || (qubit_ids && outcomes)), "null argument");
     dispatch(j, [&](auto r) { project_impl<decltype(r)>(j, count, qubit_ids, outcomes); });
+  });
+}
+
+int bqa_energy_noalloc(const double* bloch_dev, int32_t nodes_number, const double* fields_dev,
+                       int32_t edges_number, const int32_t* lhs_dev, const int32_t* rhs_dev, const double* j_dev,
+                       double* energy_dev, void* stream) {
+  return guarded([&] {
+    require(nodes_number >= 0 && edges_number >= 0, "negative size");
+    require(nodes_number <= kNoAllocMaxItems && edges_number <= kNoAllocMaxItems,
+            "more than 2^30 qubits or edges: beyond what one call of bqa_energy_noalloc serves");
+    require(energy_dev != nullptr, "null energy output");
+    require(nodes_number == 0 || (bloch_dev && fields_dev), "null Bloch or field array");
+    require(edges_number == 0 || (lhs_dev && rhs_dev && j_dev), "null edge array");
+    double* part = scratch_on_device();
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    // the same kernels and grid as bqa_energy: every partial slot is written before it is read
+    hipLaunchKernelGGL(k_energy_partial, dim3(kEnergyBlocks), dim3(256), 0, s, bloch_dev, nodes_number, fields_dev,
+                       lhs_dev, rhs_dev, j_dev, edges_number, part);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, s, part, kEnergyBlocks, energy_dev);
+    LAUNCH_CHECK();
   });
 }
 

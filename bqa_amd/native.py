@@ -44,6 +44,8 @@ SIGNATURES = {
     "bqa_density": (c_int, [c_void_p, _f64p, _f64p]),
     "bqa_energy": (c_int, [c_void_p, _f64p, POINTER(c_int8)]),
     "bqa_project": (c_int, [c_void_p, c_int32, _i32p, _i32p]),
+    "bqa_energy_noalloc": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
     "bqa_bond_dim": (c_int, [c_void_p, _i32p]),
     "bqa_get_tensors": (c_int, [c_void_p, c_int32, _f64p]),
     "bqa_set_tensors": (c_int, [c_void_p, c_int32, c_int32, _f64p]),
@@ -289,6 +291,14 @@ class Job:
         n, ms, b = c_int64(0), c_double(0.0), c_double(0.0)
         check(LIB.bqa_timing_read(self.handle, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
         return n.value, ms.value, b.value
+
+
+def energy_noalloc(bloch_dev: int, nodes_number: int, fields_dev: int, edges_number: int, lhs_dev: int,
+                   rhs_dev: int, j_dev: int, energy_dev: int, stream: int = 0) -> None:
+    """``bqa_energy_noalloc`` on device pointers (integers, e.g. torch ``data_ptr()``)."""
+    check(LIB.bqa_energy_noalloc(c_void_p(bloch_dev), int(nodes_number), c_void_p(fields_dev), int(edges_number),
+                                 c_void_p(lhs_dev), c_void_p(rhs_dev), c_void_p(j_dev), c_void_p(energy_dev),
+                                 c_void_p(stream)))
 
 
 def copy_calibrate(nbytes: int, repeats: int = 20) -> float:

@@ -143,6 +143,23 @@ int bqa_energy(bqa_job* job, double* energy, int8_t* spins_out);
  */
 int bqa_project(bqa_job* job, int32_t count, const int32_t* qubit_ids, const int32_t* outcomes);
 
+/*
+ * Allocation-free variant of bqa_energy.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The rounded-spin
+ * energy sum_i h_i s_i + sum_(ij) J_ij s_i s_j, s_i = +1 iff bloch[3 i + 2] > 0,
+ * of caller-owned device arrays: bloch_dev (3 n doubles), fields_dev (n),
+ * lhs_dev / rhs_dev / j_dev (the E forward edges); the FP64 result is written
+ * to the device double *energy_dev.  Kernels run on `stream` (a hipStream_t;
+ * NULL for the null stream) without a host synchronisation.  The block
+ * partial sums go to a 4 KB static device scratch array shared by every call
+ * on a GPU: calls that overlap on one GPU race.  n and E are at most 2^30.
+ * Same kernels and summation order as bqa_energy, so the results are equal.
+ */
+int bqa_energy_noalloc(const double* bloch_dev, int32_t nodes_number, const double* fields_dev,
+                       int32_t edges_number, const int32_t* lhs_dev, const int32_t* rhs_dev,
+                       const double* j_dev, double* energy_dev, void* stream);
+
 /* Current bond dimension. */
 int bqa_bond_dim(const bqa_job* job, int32_t* chi);
 
