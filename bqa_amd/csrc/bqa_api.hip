@@ -26,6 +26,7 @@
 #include "kernels_canon_reg.hpp"
 #include "kernels_fast.hpp"
 #include "kernels_node.hpp"
+#include "kernels_sv.hpp"
 
 using namespace bqa;
 
@@ -952,6 +953,47 @@ size_t estimate(int N, int M, const std::vector<int>& deg, const std::vector<int
 
 }  // namespace
 
+// ------------------------------------------------------ exact simulator --
+
+struct bqa_sv {
+  int n = 0, E = 0, device = 0;
+  DevBuf psi, ez, graph, part;
+  hipStream_t stream = nullptr;
+  size_t dev_bytes = 0;
+};
+
+namespace {
+
+constexpr int kSvMaxQubits = 32;
+constexpr int kSvMaxGrid = 2048;
+
+int sv_grid(unsigned long long tiles) { return (int)std::min<unsigned long long>(tiles, kSvMaxGrid); }
+
+// The X rotations (MODE 0, with the diagonal phase when ez is given) or the readout sums
+// (MODE 1) of every qubit, pass by pass; `each` gets the block count, first qubit and qubit
+// count of a readout pass.
+template <int MODE, typename F>
+void sv_passes(bqa_sv* s, const double* ez, double ztime, double xtime, F&& each) {
+  const int n = s->n;
+  const double c = std::cos(xtime), sn = std::sin(xtime);
+  auto* psi = reinterpret_cast<cplx<double>*>(s->psi.p);
+  auto* part = reinterpret_cast<double*>(s->part.p);
+  const int tb = std::min(n, kSvTileBits);
+  const int g0 = sv_grid(1ULL << (n - tb));
+  hipLaunchKernelGGL((k_sv_low<MODE>), dim3(g0), dim3(kSvThreads), 0, s->stream, psi, n, ez, ztime, c, sn, part);
+  LAUNCH_CHECK();
+  each(g0, 0, tb);
+  for (int b0 = tb; b0 < n; b0 += kSvTileBits - kSvRowBits) {
+    const int t = std::min(kSvTileBits - kSvRowBits, n - b0);
+    const int g = sv_grid(1ULL << (n - kSvRowBits - t));
+    hipLaunchKernelGGL((k_sv_high<MODE>), dim3(g), dim3(kSvThreads), 0, s->stream, psi, n, b0, t, c, sn, part);
+    LAUNCH_CHECK();
+    each(g, b0, t);
+  }
+}
+
+}  // namespace
+
 // =================================================================== C ABI
 
 extern "C" {
@@ -1365,6 +1407,127 @@ int bqa_timing_read(bqa_job* j, int64_t* launches, double* total_ms, double* byt
     *launches = j->t_launches;
     *total_ms = j->t_ms;
     *bytes = j->t_bytes;
+  });
+}
+
+int bqa_sv_create(bqa_sv** out, int32_t qubits, const double* fields, int32_t edges_number, const int32_t* lhs,
+                  const int32_t* rhs, const double* couplings) {
+  bqa_sv* s = nullptr;
+  int rc = guarded([&] {
+    require(out != nullptr, "null simulator pointer");
+    require(qubits >= 1 && qubits <= kSvMaxQubits, "the exact simulator takes 1 to 32 qubits");
+    require(edges_number >= 0 && edges_number <= kSvMaxEdges, "the exact simulator takes at most 2048 edges");
+    require(fields != nullptr && (edges_number == 0 || (lhs && rhs && couplings)), "null argument");
+    for (int e = 0; e < edges_number; ++e)
+      require(lhs[e] >= 0 && lhs[e] < qubits && rhs[e] >= 0 && rhs[e] < qubits && lhs[e] != rhs[e],
+              "edge endpoint out of range");
+    int dev_count = 0;
+    HIPX(hipGetDeviceCount(&dev_count));
+    require(dev_count > 0, "no HIP device");
+    s = new bqa_sv();
+    s->n = qubits;
+    s->E = edges_number;
+    HIPX(hipGetDevice(&s->device));
+    HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const size_t dim = (size_t)1 << qubits;
+    s->psi.ensure(dim * sizeof(cplx<double>), s->dev_bytes);
+    s->ez.ensure(dim * sizeof(double), s->dev_bytes);
+    s->part.ensure((size_t)kSvMaxGrid * 4 * kSvTileBits * sizeof(double), s->dev_bytes);
+    const size_t gb = sizeof(double) * ((size_t)qubits + edges_number) + sizeof(int32_t) * 2 * (size_t)edges_number + 64;
+    s->graph.ensure(gb, s->dev_bytes);
+    char* base = reinterpret_cast<char*>(s->graph.p);
+    double* h = reinterpret_cast<double*>(base);
+    double* jv = h + qubits;
+    int32_t* a = reinterpret_cast<int32_t*>(jv + edges_number);
+    int32_t* b = a + edges_number;
+    HIPX(hipMemcpy(h, fields, sizeof(double) * qubits, hipMemcpyHostToDevice));
+    if (edges_number) {
+      HIPX(hipMemcpy(jv, couplings, sizeof(double) * edges_number, hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(a, lhs, sizeof(int32_t) * edges_number, hipMemcpyHostToDevice));
+      HIPX(hipMemcpy(b, rhs, sizeof(int32_t) * edges_number, hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(k_sv_energy, dim3(grid_for((long)std::min<size_t>(dim, 1L << 30), kSvThreads, 16384)),
+                       dim3(kSvThreads), 0, s->stream, reinterpret_cast<double*>(s->ez.p), qubits, h, edges_number, a,
+                       b, jv);
+    LAUNCH_CHECK();
+    HIPX(hipStreamSynchronize(s->stream));
+    *out = s;
+  });
+  if (rc != BQA_OK && s) bqa_sv_destroy(s);
+  return rc;
+}
+
+int bqa_sv_destroy(bqa_sv* s) {
+  if (!s) return BQA_OK;
+  return guarded([&] {
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->psi.release(s->dev_bytes);
+    s->ez.release(s->dev_bytes);
+    s->graph.release(s->dev_bytes);
+    s->part.release(s->dev_bytes);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+  });
+}
+
+int bqa_sv_init(bqa_sv* s) {
+  return guarded([&] {
+    require(s, "null simulator");
+    HIPX(hipSetDevice(s->device));
+    const long dim = 1L << s->n;
+    hipLaunchKernelGGL(k_sv_init, dim3(grid_for(std::min(dim, 1L << 30), 256, 16384)), dim3(256), 0, s->stream,
+                       reinterpret_cast<cplx<double>*>(s->psi.p), s->n);
+    LAUNCH_CHECK();
+    HIPX(hipStreamSynchronize(s->stream));
+  });
+}
+
+int bqa_sv_run_layer(bqa_sv* s, double xtime, double ztime) {
+  return guarded([&] {
+    require(s, "null simulator");
+    HIPX(hipSetDevice(s->device));
+    sv_passes<0>(s, reinterpret_cast<const double*>(s->ez.p), ztime, xtime, [](int, int, int) {});
+    HIPX(hipStreamSynchronize(s->stream));
+  });
+}
+
+int bqa_sv_density(bqa_sv* s, double* rho_out, double* bloch_out) {
+  return guarded([&] {
+    require(s, "null simulator");
+    HIPX(hipSetDevice(s->device));
+    DevBuf res;
+    size_t scratch_bytes = 0;
+    res.ensure((size_t)s->n * 11 * sizeof(double), scratch_bytes);
+    double* rho = reinterpret_cast<double*>(res.p);
+    double* bloch = rho + 8 * (size_t)s->n;
+    try {
+      sv_passes<1>(s, nullptr, 0.0, 0.0, [&](int blocks, int q0, int nb) {
+        hipLaunchKernelGGL(k_sv_finish, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<const double*>(s->part.p),
+                           blocks, q0, nb, rho, bloch);
+        LAUNCH_CHECK();
+      });
+      if (rho_out)
+        HIPX(hipMemcpyAsync(rho_out, rho, (size_t)s->n * 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      if (bloch_out)
+        HIPX(hipMemcpyAsync(bloch_out, bloch, (size_t)s->n * 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      HIPX(hipStreamSynchronize(s->stream));
+    } catch (...) {
+      (void)hipStreamSynchronize(s->stream);
+      res.release(scratch_bytes);
+      throw;
+    }
+    res.release(scratch_bytes);
+  });
+}
+
+int bqa_sv_get_state(bqa_sv* s, double* out) {
+  return guarded([&] {
+    require(s && out, "null argument");
+    require(s->n <= 26, "bqa_sv_get_state copies at most 2^26 amplitudes");
+    HIPX(hipSetDevice(s->device));
+    HIPX(hipMemcpyAsync(out, s->psi.p, ((size_t)1 << s->n) * sizeof(cplx<double>), hipMemcpyDeviceToHost, s->stream));
+    HIPX(hipStreamSynchronize(s->stream));
   });
 }
 

@@ -56,6 +56,12 @@ SIGNATURES = {
     "bqa_timing_enable": (c_int, [c_void_p, c_int32]),
     "bqa_timing_read": (c_int, [c_void_p, POINTER(c_int64), _f64p, _f64p]),
     "bqa_copy_calibrate": (c_int, [c_uint64, c_int32, _f64p]),
+    "bqa_sv_create": (c_int, [POINTER(c_void_p), c_int32, _f64p, c_int32, _i32p, _i32p, _f64p]),
+    "bqa_sv_destroy": (c_int, [c_void_p]),
+    "bqa_sv_init": (c_int, [c_void_p]),
+    "bqa_sv_run_layer": (c_int, [c_void_p, c_double, c_double]),
+    "bqa_sv_density": (c_int, [c_void_p, _f64p, _f64p]),
+    "bqa_sv_get_state": (c_int, [c_void_p, _f64p]),
 }
 
 
@@ -305,3 +311,47 @@ def copy_calibrate(nbytes: int, repeats: int = 20) -> float:
     ms = c_double(0.0)
     check(LIB.bqa_copy_calibrate(int(nbytes), int(repeats), ctypes.byref(ms)))
     return ms.value
+
+
+class StateVector:
+    """The exact state-vector validator on the GPU (a ``bqa_sv`` handle, include/bqa_hip.h)."""
+
+    def __init__(self, qubits: int, fields, lhs, rhs, couplings):
+        self.handle = None
+        self.qubits = int(qubits)
+        h = _f64(fields)
+        a, b, j = _i32(lhs), _i32(rhs), _f64(couplings)
+        if h.shape != (self.qubits,) or not (a.shape == b.shape == j.shape):
+            raise ValueError("fields must have one entry per qubit and the edge arrays one per edge")
+        hd = c_void_p()
+        check(LIB.bqa_sv_create(ctypes.byref(hd), self.qubits, _p(h, c_double), int(a.size), _p(a, c_int32),
+                                _p(b, c_int32), _p(j, c_double)))
+        self.handle = hd
+
+    def close(self) -> None:
+        if self.handle is not None and self.handle.value:
+            check(LIB.bqa_sv_destroy(self.handle))
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_state(self) -> None:
+        check(LIB.bqa_sv_init(self.handle))
+
+    def run_layer(self, xtime: float, ztime: float) -> None:
+        check(LIB.bqa_sv_run_layer(self.handle, float(xtime), float(ztime)))
+
+    def density(self) -> tuple[np.ndarray, np.ndarray]:
+        rho = np.empty((self.qubits, 2, 2), dtype=np.complex128)
+        bloch = np.empty((self.qubits, 3), dtype=np.float64)
+        check(LIB.bqa_sv_density(self.handle, rho.ctypes.data_as(_f64p), bloch.ctypes.data_as(_f64p)))
+        return rho, bloch
+
+    def state(self) -> np.ndarray:
+        out = np.empty(1 << self.qubits, dtype=np.complex128)
+        check(LIB.bqa_sv_get_state(self.handle, out.ctypes.data_as(_f64p)))
+        return out

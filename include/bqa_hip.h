@@ -172,6 +172,28 @@ int bqa_set_messages(bqa_job* job, int32_t chi, const double* in);
 int bqa_get_lambdas(bqa_job* job, double* out);
 int bqa_set_lambdas(bqa_job* job, int32_t chi, const double* in);
 
+/*
+ * Exact state-vector validator on the GPU (replaces src/bqa/exact_sim.py:14-88,
+ * whose simulator comes from the qem package): the same Trotterised circuit on
+ * the full 2^n amplitude vector in FP64, for n <= 32 qubits (24 GB of device
+ * memory at 30 qubits).  Qubit q is bit q of the basis index.
+ *   bqa_sv_create    -- fields h (n), the E undirected edges (lhs, rhs, J); E <= 2048
+ *   bqa_sv_init      -- |->^n (exact_sim.py:45-49)
+ *   bqa_sv_run_layer -- exp(-i ztime h_q Z_q) and exp(-i ztime J_ab Z_a Z_b) on every
+ *                       qubit and edge, then exp(-i xtime X_q) on every qubit (:51-63)
+ *   bqa_sv_density   -- per-qubit reduced density matrices (n*8 doubles) and Bloch
+ *                       vectors (n*3 doubles) as in bqa_density; either may be NULL (:69-70)
+ *   bqa_sv_get_state -- the amplitudes (2^n complex, n <= 26), for testing
+ */
+typedef struct bqa_sv bqa_sv;
+int bqa_sv_create(bqa_sv** sv, int32_t qubits, const double* fields, int32_t edges_number,
+                  const int32_t* lhs, const int32_t* rhs, const double* couplings);
+int bqa_sv_destroy(bqa_sv* sv);
+int bqa_sv_init(bqa_sv* sv);
+int bqa_sv_run_layer(bqa_sv* sv, double xtime, double ztime);
+int bqa_sv_density(bqa_sv* sv, double* rho_out, double* bloch_out);
+int bqa_sv_get_state(bqa_sv* sv, double* out);
+
 /* Kernel timing of the BP sweep kernels, recorded with HIP events on the job's
  * stream while enabled: launches, total milliseconds, algorithmic bytes moved. */
 int bqa_timing_enable(bqa_job* job, int32_t on);
