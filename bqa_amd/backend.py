@@ -188,6 +188,12 @@ class MI355XBackend:
     def set_symmetric_gauge(self):              # state.py:219-227
         self.job.set_symmetric_gauge()
 
+    def set_vidal_gauge(self, pinv_eps):        # state.py:204-216
+        self.job.set_vidal_gauge(pinv_eps)
+
+    def truncate_vidal_gauge(self, pinv_eps):   # state.py:160-168
+        return self.job.truncate_vidal_gauge(pinv_eps)
+
     def run_layer(self, xtime, ztime, pinv_eps, max_iters, bp_eps, damping):   # state.py:315-321
         return self.job.run_layer(xtime, ztime, pinv_eps, max_iters, bp_eps, damping)
 

@@ -645,6 +645,8 @@ void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64
   LAUNCH_CHECK();
 }
 
+void canonicalise(bqa_job* j, double pinv_eps);
+
 template <typename R>
 void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
   const int chi = j->chi, n2 = 2 * chi;
@@ -678,6 +680,13 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
     if (c.d == 0 || c.n == 0) continue;
     ext_class<R>(j, c, ztime, msgs64);
   }
+  canonicalise(j, pinv_eps);
+}
+
+// FP64 canonicalisers and lambdas of the 2chi x 2chi messages in j->ext (state.py:171-200), then
+// the per-column lambda maxima on the host (the global truncation rule reads them).
+void canonicalise(bqa_job* j, double pinv_eps) {
+  const int chi = j->chi, n2 = 2 * chi;
   const int ccols = std::min(n2, j->chimax);
   j->canon.ensure((size_t)j->M * n2 * ccols * sizeof(zd), j->dev_bytes, j->stream);
   j->lmbd.ensure((size_t)j->E * n2 * sizeof(double), j->dev_bytes, j->stream);
@@ -765,6 +774,109 @@ void simple_update_impl(bqa_job* j, double ztime, double pinv_eps, int extra_fla
   j->lstride = n2;
   local_update<R>(j, kAbsorb | extra_flags, dim, ztime, z_time, x_time);
   *new_dim = dim;
+}
+
+// Messages as the top-left chi x chi block of otherwise zero 2chi x 2chi matrices: the s = 0
+// block of the extended-message layout, so the canonicalisers of these matrices are the
+// canonicalisers of the messages in their first chi rows and zero below.
+template <typename R>
+__global__ void k_pad_msgs(const cplx<R>* __restrict__ m, cplx<double>* __restrict__ ext, long M, int chi) {
+  const int n2 = 2 * chi;
+  const long per = (long)n2 * n2;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < M * per; x += (long)gridDim.x * blockDim.x) {
+    const long pos = x / per;
+    const int o = (int)(x - pos * per), r = o / n2, c = o % n2;
+    ext[x] = (r < chi && c < chi) ? cast<double>(m[pos * chi * chi + r * chi + c]) : mk(0.0, 0.0);
+  }
+}
+
+// Tensors (n, 2, chi_in^d) -> (n, 2, chi_out^d) keeping indices below chi_out on every bond.
+template <typename R>
+__global__ void k_slice_tensors(const cplx<R>* __restrict__ Tin, cplx<R>* __restrict__ Tout, long n, int d,
+                                int chi_in, int chi_out) {
+  long a_in = 1, a_out = 1;
+  for (int k = 0; k < d; ++k) {
+    a_in *= chi_in;
+    a_out *= chi_out;
+  }
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < n * 2 * a_out; x += (long)gridDim.x * blockDim.x) {
+    const long np = x / a_out;           // node * 2 + p
+    long a = x - np * a_out, src = 0, mul = 1;
+    for (int k = 0; k < d; ++k) {        // last bond fastest
+      const long digit = a % chi_out;
+      a /= chi_out;
+      src += digit * mul;
+      mul *= chi_in;
+    }
+    Tout[x] = Tin[np * a_in + src];
+  }
+}
+
+// Messages (M, chi_in, chi_in) -> their leading (chi_out, chi_out) blocks.
+template <typename R>
+__global__ void k_slice_msgs(const cplx<R>* __restrict__ min, cplx<R>* __restrict__ mout, long M, int chi_in,
+                             int chi_out) {
+  const long per = (long)chi_out * chi_out;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < M * per; x += (long)gridDim.x * blockDim.x) {
+    const long pos = x / per;
+    const int o = (int)(x - pos * per), r = o / chi_out, c = o % chi_out;
+    mout[x] = min[pos * chi_in * chi_in + r * chi_in + c];
+  }
+}
+
+// Vidal gauge (state.py:204-216): canonicalisers of the current messages, absorbed into every
+// tensor bond without a ZZ extension (zz_time 0 makes the extension the identity on the s = 0
+// block, the only non-zero rows); the bond dimension and the messages stay as they are.
+template <typename R>
+void vidal_impl(bqa_job* j, double pinv_eps) {
+  const int chi = j->chi, n2 = 2 * chi;
+  j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
+  const long cnt = (long)j->M * n2 * n2;
+  hipLaunchKernelGGL((k_pad_msgs<R>), dim3(grid_for(cnt)), dim3(256), 0, j->stream, msg_ptr<R>(j, j->mcur),
+                     reinterpret_cast<cplx<double>*>(j->ext.p), (long)j->M, chi);
+  LAUNCH_CHECK();
+  canonicalise(j, pinv_eps);
+  j->lstride = n2;
+  j->zz_valid = false;
+  local_update<R>(j, kAbsorb, chi, 0.0, 0.0, 0.0);
+}
+
+// state.py:160-168: the global truncation rule on the lambdas (backends.py:297-303) and every
+// tensor bond cut to the new bond dimension.  The reference leaves its messages at the old size,
+// which its own BP cannot then read; here they keep their leading block, so the job stays whole.
+template <typename R>
+void truncate_vidal_impl(bqa_job* j, double pinv_eps, int* dim, double* err) {
+  const int chi = j->chi, ls = j->lstride;
+  unsigned long long* lm = reinterpret_cast<unsigned long long*>(j->lmax.p);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, j->stream, lm, ls);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_lambda_max, dim3(grid_for((long)j->E * ls, 256, 1024)), dim3(256), 0, j->stream,
+                     reinterpret_cast<const double*>(j->lmbd.p), j->E, ls, lm);
+  LAUNCH_CHECK();
+  HIPX(hipMemcpyAsync(j->lmax_host, lm, sizeof(double) * ls, hipMemcpyDeviceToHost, j->stream));
+  HIPX(hipStreamSynchronize(j->stream));
+  const int d_new = std::min(truncation_dim(j, ls, pinv_eps, err), chi);
+  if (d_new < chi) {
+    for (auto& c : j->cls) {
+      if (c.n == 0) continue;
+      const int dst = 1 - c.tcur;
+      ensure_tensor(j, c, dst, d_new);
+      const long total = (long)c.n * 2 * ipow_l(d_new, c.d);
+      hipLaunchKernelGGL((k_slice_tensors<R>), dim3(grid_for(total)), dim3(256), 0, j->stream,
+                         reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), reinterpret_cast<cplx<R>*>(c.T[dst].p),
+                         (long)c.n, c.d, chi, d_new);
+      LAUNCH_CHECK();
+      c.tcur = dst;
+    }
+    const long mc = (long)j->M * d_new * d_new;
+    hipLaunchKernelGGL((k_slice_msgs<R>), dim3(grid_for(mc)), dim3(256), 0, j->stream, msg_ptr<R>(j, j->mcur),
+                       msg_ptr<R>(j, 1 - j->mcur), (long)j->M, chi, d_new);
+    LAUNCH_CHECK();
+    j->mcur = 1 - j->mcur;
+    j->chi = d_new;
+  }
+  HIPX(hipStreamSynchronize(j->stream));
+  *dim = d_new;
 }
 
 template <typename R>
@@ -1209,6 +1321,24 @@ int bqa_simple_update(bqa_job* j, double ztime, double pinv_eps, int32_t* new_bo
     dispatch(j, [&](auto r) {
       int dim = 0;
       simple_update_impl<decltype(r)>(j, ztime, pinv_eps, 0, 0.0, 0.0, &dim, trunc_err);
+      *new_bond_dim = dim;
+    });
+  });
+}
+
+int bqa_set_vidal_gauge(bqa_job* j, double pinv_eps) {
+  return guarded([&] {
+    require(j, "null job");
+    dispatch(j, [&](auto r) { vidal_impl<decltype(r)>(j, pinv_eps); });
+  });
+}
+
+int bqa_truncate_vidal_gauge(bqa_job* j, double pinv_eps, int32_t* new_bond_dim, double* trunc_err) {
+  return guarded([&] {
+    require(j && new_bond_dim && trunc_err, "null argument");
+    dispatch(j, [&](auto r) {
+      int dim = 0;
+      truncate_vidal_impl<decltype(r)>(j, pinv_eps, &dim, trunc_err);
       *new_bond_dim = dim;
     });
   });

@@ -38,6 +38,8 @@ SIGNATURES = {
     "bqa_run_bp": (c_int, [c_void_p, c_int32, c_double, c_double, _i32p, _i32p, _f64p]),
     "bqa_simple_update": (c_int, [c_void_p, c_double, c_double, _i32p, _f64p]),
     "bqa_apply_local_gates": (c_int, [c_void_p, c_double, c_double]),
+    "bqa_set_vidal_gauge": (c_int, [c_void_p, c_double]),
+    "bqa_truncate_vidal_gauge": (c_int, [c_void_p, c_double, _i32p, _f64p]),
     "bqa_set_symmetric_gauge": (c_int, [c_void_p]),
     "bqa_run_layer": (c_int, [c_void_p, c_double, c_double, c_double, c_int32, c_double, c_double, _i32p, _f64p,
                               _i32p, _i32p, _f64p]),
@@ -216,6 +218,14 @@ class Job:
 
     def set_symmetric_gauge(self) -> None:
         check(LIB.bqa_set_symmetric_gauge(self.handle))
+
+    def set_vidal_gauge(self, pinv_eps: float) -> None:
+        check(LIB.bqa_set_vidal_gauge(self.handle, float(pinv_eps)))
+
+    def truncate_vidal_gauge(self, pinv_eps: float) -> tuple[int, float]:
+        chi, err = c_int32(0), c_double(0.0)
+        check(LIB.bqa_truncate_vidal_gauge(self.handle, float(pinv_eps), ctypes.byref(chi), ctypes.byref(err)))
+        return chi.value, err.value
 
     def run_layer(self, xtime: float, ztime: float, pinv_eps: float, max_iters: int, bp_eps: float,
                   damping: float) -> dict:

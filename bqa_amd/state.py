@@ -84,6 +84,18 @@ def _set_to_symmetric_gauge(context, state: State) -> None:
     state.backend.set_symmetric_gauge()
 
 
+def _set_to_vidal_gauge(context, state: State) -> None:
+    """src/bqa/state.py:204-216 (unused by the reference's run_qa; kept for the gauge checks)."""
+    state.backend.set_vidal_gauge(context.pinv_eps)
+    log.debug("State has been set to Vidal gauge")
+
+
+def _truncate_vidal_gauge(context, state: State) -> None:
+    """src/bqa/state.py:160-168."""
+    chi, err = state.backend.truncate_vidal_gauge(context.pinv_eps)
+    log.info(f"Truncation performed, per edge error upper bound: {err}")
+
+
 def run_layer(context, xtime: float, ztime: float, state: State, fused: bool = True) -> None:
     """One Trotter step (src/bqa/state.py:315-321)."""
     if fused:

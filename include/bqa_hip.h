@@ -102,6 +102,24 @@ int bqa_run_bp(bqa_job* job, int32_t max_iters, double bp_eps, double damping,
 int bqa_simple_update(bqa_job* job, double ztime, double pinv_eps,
                       int32_t* new_bond_dim, double* trunc_err);
 
+/*
+ * Vidal gauge (src/bqa/state.py:204-216): lambdas and bond canonicalisers of
+ * the current chi x chi messages (FP64, the canonicaliser of bqa_simple_update,
+ * state.py:171-200), every tensor bond contracted with the canonicaliser at its
+ * input message position (backends.py:410-414).  The bond dimension and the
+ * messages stay as they are; bqa_get_lambdas returns the (E, chi) lambdas.
+ */
+int bqa_set_vidal_gauge(bqa_job* job, double pinv_eps);
+
+/*
+ * Truncation of a Vidal-gauge state (src/bqa/state.py:160-168): the global
+ * bond dimension chi' = min(chi_max, #{k: max_e lambda_e[k] >= pinv_eps})
+ * (backends.py:297-303), every tensor bond and the lambdas cut to chi'.  The
+ * reference leaves its messages at the old size; here they keep their leading
+ * chi' x chi' block.  Outputs: *new_bond_dim, *trunc_err.
+ */
+int bqa_truncate_vidal_gauge(bqa_job* job, double pinv_eps, int32_t* new_bond_dim, double* trunc_err);
+
 /* Local Z layer exp(-i ztime h Z) then X layer exp(-i xtime X) on every
  * qubit (src/bqa/state.py:142-156).  Pass 0 to skip either. */
 int bqa_apply_local_gates(bqa_job* job, double ztime, double xtime);

@@ -483,6 +483,30 @@ def simple_update(ctx: Context, st: State, ztime: float) -> None:
     st.trunc_errs.append(err)
 
 
+def set_to_vidal_gauge(ctx: Context, st: State) -> None:
+    """Vidal gauge from the current messages (src/bqa/state.py:204-216): lambdas and canonicalisers of the
+    chi x chi messages (state.py:171-200); every bond k of every tensor is contracted with the canonicaliser
+    at its input message position (backends.py:410-414).  Messages are left as they are."""
+    lam, canon = canonicalizers(st.msgs, ctx.pinv_eps)
+    st.lmbds = lam
+    for d in st.tensors:
+        lay = ctx.classes[d]
+        for k in range(d):
+            st.tensors[d] = right_product(st.tensors[d], canon[lay.input_pos[k]], k + 2)
+
+
+def truncate_vidal_gauge(ctx: Context, st: State) -> float:
+    """src/bqa/state.py:160-168: the global truncation rule on the lambdas (backends.py:297-303) and every
+    bond of every tensor cut to the new bond dimension (batch_truncate_all_but, backends.py:289-295).
+    Returns the truncation error."""
+    lam, chi, err = truncate_lmbds(st.lmbds, ctx.max_bond_dim, ctx.pinv_eps)
+    for d, t in st.tensors.items():
+        st.tensors[d] = t[(slice(None), slice(None)) + (slice(0, chi),) * d].copy()
+    st.lmbds = lam
+    st.chi = chi
+    return err
+
+
 def apply_z_layer(ctx: Context, st: State, ztime: float) -> None:
     """T <- (cos phi - i sin phi Z) T, phi = h ztime (src/bqa/state.py:142-150)."""
     for d, t in st.tensors.items():

@@ -3,7 +3,9 @@
 * BP on a tree converges to the exact branch environments for any chi
   (reference test_core_subroutines.py:191-204 checks chi = 1 only);
 * BP densities on a tree equal those of the fully contracted state (:207-220);
-* Vidal lambdas equal the Schmidt values across every edge (:141-158, 223-256);
+* Vidal lambdas equal the Schmidt values across every edge (:141-158, 223-256), and the
+  Vidal-gauge state (tensors with lambda on each edge) is the same state with identity
+  branch environments (test_vg_and_sg_small_tree_circuit, :223-256);
 * gate identities (test_gpt_generated_gates_application.py:26-101) and the
   ZZ split identity g0^2 I + g1^2 ZZ = exp(-i theta ZZ);
 * end to end against the exact state vector (test_small_circuit_final_density.py:9-20);
@@ -181,3 +183,76 @@ def test_z2_symmetric_instance_has_zero_magnetisation():
     orc = np.array(O.run_qa(cfg)[0][1])
     assert np.abs(ex[:, 2]).max() < 1e-12
     assert np.abs(orc[:, 2]).max() < 1e-8
+
+
+def vidal_branch(graph, tensors, lmbds_of, u, v):
+    """m_{u->v} of the Vidal-gauge state: u's branch with lambda on every edge inside it
+    (test_core_subroutines.py:37-70 with is_vidal_gauge)."""
+    from treeutil import side_nodes
+    scaled = {}
+    nodes = side_nodes(graph, u, v)
+    for w in nodes:
+        t = tensors[w]
+        for k, x in enumerate(graph[w]):
+            if x in nodes and w < x:     # each inner edge scaled once
+                lam = lmbds_of(w, x)
+                shape = [1] * t.ndim
+                shape[k + 1] = lam.size
+                t = t * lam.reshape(shape)
+        scaled[w] = t
+    full = [scaled.get(w, tensors[w]) for w in range(len(graph))]
+    return branch_message(graph, full, u, v)
+
+
+@pytest.mark.parametrize("chi", [2, 3])
+def test_vidal_gauge_on_a_tree(chi):
+    """test_vg_and_sg_small_tree_circuit restated: after BP and the Vidal gauge, lambdas are the Schmidt
+    values, every branch environment (with lambdas inside) is the identity / rank on the kept block, the
+    Vidal-gauge state with lambda on each edge is the original state, and the symmetric gauge then BP
+    gives back the same densities."""
+    ctx, st = _tree_state(chi, 45 + chi)
+    O.run_bp(ctx, st)
+    psi = full_state(ctx.graph, per_node(ctx, st.tensors))
+    rho0 = O.get_density_matrices(ctx, st)
+    O.set_to_vidal_gauge(ctx, st)
+    tens = per_node(ctx, st.tensors)
+    lam_of = lambda a, b: st.lmbds[ctx.edge_to_msg_pos[(a, b)] % ctx.lmbds_number].real
+    for (u, v), pos in ctx.edge_to_msg_pos.items():
+        want = schmidt_values(ctx.graph, psi, u, v)
+        lam = lam_of(u, v)
+        r = int(np.sum(want > 1e-9))
+        assert np.abs(lam[:r] - want[:r]).max() < 1e-9 and np.abs(lam[r:]).max(initial=0.0) < 1e-9
+        env = vidal_branch(ctx.graph, tens, lam_of, u, v)
+        assert np.abs(env[:r, :r] - np.eye(r) / r).max() < 1e-9
+        assert np.abs(env[r:]).max(initial=0.0) < 1e-9 and np.abs(env[:, r:]).max(initial=0.0) < 1e-9
+    # Vidal-gauge state: one lambda per edge
+    scaled = list(tens)
+    done = set()
+    for v in range(ctx.nodes_number):
+        for k, w in enumerate(ctx.graph[v]):
+            if (w, v) in done:
+                continue
+            done.add((v, w))
+            shape = [1] * scaled[v].ndim
+            shape[k + 1] = chi
+            scaled[v] = scaled[v] * lam_of(v, w).reshape(shape)
+    psi_vg = full_state(ctx.graph, scaled)
+    ov = abs(np.vdot(psi, psi_vg)) / (np.linalg.norm(psi) * np.linalg.norm(psi_vg))
+    assert abs(ov - 1.0) < 1e-10
+    O.set_symmetric_gauge(ctx, st)
+    O.run_bp(ctx, st)
+    assert np.abs(O.get_density_matrices(ctx, st) - rho0).max() < 1e-10
+
+
+def test_truncate_vidal_gauge_keeps_the_global_rank():
+    """The truncation after the Vidal gauge keeps min(max_bond_dim, rank) columns on every bond."""
+    ctx, st = _tree_state(3, 61)
+    O.run_bp(ctx, st)
+    O.set_to_vidal_gauge(ctx, st)
+    lam = st.lmbds.real.copy()
+    ctx.max_bond_dim = 2
+    err = O.truncate_vidal_gauge(ctx, st)
+    assert st.chi == 2 and st.lmbds.shape == (ctx.lmbds_number, 2)
+    for d, t in st.tensors.items():
+        assert t.shape[2:] == (2,) * d
+    assert abs(err - np.sqrt(np.sum(lam.max(0)[2:] ** 2))) < 1e-12
