@@ -23,7 +23,6 @@
 #include "common.hpp"
 #include "kernels_canon.hpp"
 #include "kernels_canon_h.hpp"
-#include "kernels_canon_reg.hpp"
 #include "kernels_fast.hpp"
 #include "kernels_node.hpp"
 #include "kernels_sv.hpp"
@@ -180,7 +179,6 @@ struct bqa_job {
   // any of the letters b (BP sweep), d (density), e (extended messages),
   // c (canonicaliser), l (absorb/gates/gauge) selects them per family.
   unsigned generic_mask = 0;
-  bool canon_full_columns = false;  // BQA_CANON_REG=1: full-column register canonicaliser
   bool generic(unsigned fam) const { return (generic_mask & fam) != 0; }
 
   bool single() const { return prec == BQA_PREC_SINGLE; }
@@ -554,18 +552,6 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
   if (!h.have_best) throw BqaError(BQA_ERR_NUMERIC, "BP produced no finite message distance");
 }
 
-template <typename R, int N>
-void launch_canon_reg(bqa_job* j, int ccols, double pinv_eps, double deps) {
-  using S = RegCanonShape<N>;
-  j->cws.ensure((size_t)j->E * S::WS * sizeof(double), j->dev_bytes, j->stream);
-  const int grid = (j->E + S::EPW - 1) / S::EPW;
-  hipLaunchKernelGGL((k_canon_reg<R, N>), dim3(grid), dim3(64), 0, j->stream,
-                     reinterpret_cast<const cplx<R>*>(j->ext.p), j->E, ccols, pinv_eps, deps,
-                     reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p),
-                     reinterpret_cast<double*>(j->cws.p));
-  LAUNCH_CHECK();
-}
-
 template <int N>
 void launch_canon_h(bqa_job* j, int ccols, double pinv_eps, double deps) {
   using S = HCanon<N>;
@@ -578,32 +564,17 @@ void launch_canon_h(bqa_job* j, int ccols, double pinv_eps, double deps) {
   LAUNCH_CHECK();
 }
 
-// Register-resident canonicalisers for chi <= 8 (split-column Cholesky + Jacobi kernel unless
-// BQA_CANON_REG=1 selects the full-column kernel); false for larger chi.
-template <typename R>
+// Register-resident canonicalisers for chi <= 8 (split-column Cholesky + Jacobi); false for larger chi.
 bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double deps) {
-  if (!j->canon_full_columns) {
-    switch (chi) {
-      case 1: launch_canon_h<2>(j, ccols, pinv_eps, deps); return true;
-      case 2: launch_canon_h<4>(j, ccols, pinv_eps, deps); return true;
-      case 3: launch_canon_h<6>(j, ccols, pinv_eps, deps); return true;
-      case 4: launch_canon_h<8>(j, ccols, pinv_eps, deps); return true;
-      case 5: launch_canon_h<10>(j, ccols, pinv_eps, deps); return true;
-      case 6: launch_canon_h<12>(j, ccols, pinv_eps, deps); return true;
-      case 7: launch_canon_h<14>(j, ccols, pinv_eps, deps); return true;
-      case 8: launch_canon_h<16>(j, ccols, pinv_eps, deps); return true;
-      default: return false;
-    }
-  }
   switch (chi) {
-    case 1: launch_canon_reg<R, 2>(j, ccols, pinv_eps, deps); return true;
-    case 2: launch_canon_reg<R, 4>(j, ccols, pinv_eps, deps); return true;
-    case 3: launch_canon_reg<R, 6>(j, ccols, pinv_eps, deps); return true;
-    case 4: launch_canon_reg<R, 8>(j, ccols, pinv_eps, deps); return true;
-    case 5: launch_canon_reg<R, 10>(j, ccols, pinv_eps, deps); return true;
-    case 6: launch_canon_reg<R, 12>(j, ccols, pinv_eps, deps); return true;
-    case 7: launch_canon_reg<R, 14>(j, ccols, pinv_eps, deps); return true;
-    case 8: launch_canon_reg<R, 16>(j, ccols, pinv_eps, deps); return true;
+    case 1: launch_canon_h<2>(j, ccols, pinv_eps, deps); return true;
+    case 2: launch_canon_h<4>(j, ccols, pinv_eps, deps); return true;
+    case 3: launch_canon_h<6>(j, ccols, pinv_eps, deps); return true;
+    case 4: launch_canon_h<8>(j, ccols, pinv_eps, deps); return true;
+    case 5: launch_canon_h<10>(j, ccols, pinv_eps, deps); return true;
+    case 6: launch_canon_h<12>(j, ccols, pinv_eps, deps); return true;
+    case 7: launch_canon_h<14>(j, ccols, pinv_eps, deps); return true;
+    case 8: launch_canon_h<16>(j, ccols, pinv_eps, deps); return true;
     default: return false;
   }
 }
@@ -693,7 +664,7 @@ void canonicalise(bqa_job* j, double pinv_eps) {
   const size_t wsb = canon_ws_bytes(n2);
   const double deps = j->single() ? (double)std::numeric_limits<float>::epsilon()
                                   : std::numeric_limits<double>::epsilon();
-  if (j->generic(kGenCanon) || !canon_reg_dispatch<double>(j, chi, ccols, pinv_eps, deps)) {
+  if (j->generic(kGenCanon) || !canon_reg_dispatch(j, chi, ccols, pinv_eps, deps)) {
     int grid;
     size_t lds;
     unsigned char* ws = nullptr;
@@ -1171,8 +1142,6 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
       for (const char* q = g; q && *q; ++q)
         j->generic_mask |= *q == 'b' ? kGenBP : *q == 'd' ? kGenDensity : *q == 'e' ? kGenExt
                          : *q == 'c' ? kGenCanon : *q == 'l' ? kGenLocal : 0u;
-      const char* c = std::getenv("BQA_CANON_REG");
-      j->canon_full_columns = c && c[0] == '1';
     }
     long total_nodes = 0, total_bonds = 0;
     for (int c = 0; c < classes; ++c) {

@@ -1,6 +1,6 @@
 // Canonicalisers with split columns for bond dimensions chi <= 8 (n = 2 chi <= 16).
 //
-// Same results as k_canonicalize / k_canon_reg, organised for occupancy and
+// Same results as k_canonicalize, organised for occupancy and
 // fewer Jacobi sweeps:
 //   * 2 * P2 lanes per edge (P2 = n/2 rounded up to a power of two): lane
 //     (k, h) = k + P2 h holds rows [h n/2, (h+1) n/2) of the two columns of

@@ -3,8 +3,6 @@
 // runs the generic kernel of kernels_node.hpp.
 #pragma once
 
-#include <cstdlib>
-
 #include "common.hpp"
 #include "kernels_node.hpp"
 #include "kernels_mfma.hpp"
@@ -97,37 +95,17 @@ bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msg
   return launch_small<kModeExt, R, double>(cv, chi, T, m, m, nullptr, ext, nullptr, nullptr, s);
 }
 
-// FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp).
-// k_mfma8: one block of W waves per node (degree 3: W = 1, 2 or 4; degree 2: W = 1).
-inline int mfma8_waves() {
-  static const int w = [] {
-    const char* e = std::getenv("BQA_MFMA8_WAVES");
-    const int v = e ? std::atoi(e) : 1;
-    return v == 2 || v == 4 ? v : 1;
-  }();
-  return w;
-}
-
-template <int MODE, int W>
-inline void launch_mfma8_3(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
-                           double* rho, double* bloch, hipStream_t s) {
-  hipLaunchKernelGGL((k_mfma8<3, MODE, W>), dim3(node_grid<k_mfma8<3, MODE, W>, 64 * W>(cv.n)), dim3(64 * W), 0, s,
-                     cv, T, mA, mB, st, rho, bloch);
-}
-
+// FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp), one wave per node (two and
+// four waves per node for degree 3 measured no faster).
 template <int MODE>
 inline void launch_mfma8(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
                          double* rho, double* bloch, hipStream_t s) {
-  if (cv.d == 2) {
+  if (cv.d == 2)
     hipLaunchKernelGGL((k_mfma8<2, MODE, 1>), dim3(node_grid<k_mfma8<2, MODE, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
                        mA, mB, st, rho, bloch);
-    return;
-  }
-  switch (mfma8_waves()) {
-    case 4: launch_mfma8_3<MODE, 4>(cv, T, mA, mB, st, rho, bloch, s); break;
-    case 2: launch_mfma8_3<MODE, 2>(cv, T, mA, mB, st, rho, bloch, s); break;
-    default: launch_mfma8_3<MODE, 1>(cv, T, mA, mB, st, rho, bloch, s); break;
-  }
+  else
+    hipLaunchKernelGGL((k_mfma8<3, MODE, 1>), dim3(node_grid<k_mfma8<3, MODE, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
+                       mA, mB, st, rho, bloch);
 }
 
 template <>
