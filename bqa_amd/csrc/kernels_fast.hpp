@@ -6,6 +6,7 @@
 #include "common.hpp"
 #include "kernels_node.hpp"
 #include "kernels_mfma.hpp"
+#include "kernels_mfma16.hpp"
 #include "kernels_small.hpp"
 
 namespace bqa {
@@ -38,6 +39,23 @@ template <auto K, int THREADS>
 inline int node_grid(int n) {
   const int r = resident_blocks<K, THREADS>();
   return n < r ? n : r;
+}
+
+// Kernels with more than 64 KB of dynamic LDS: the size is allowed once per kernel, and the
+// resident-block count takes it into account.
+template <auto K, int THREADS, int LDS>
+inline int node_grid_lds(int n) {
+  static const int blocks = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (LDS > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(K),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, K, THREADS, LDS) != hipSuccess || per <= 0) per = 1;
+    return cus * per;
+  }();
+  return n < blocks ? n : blocks;
 }
 
 // Compile-time-shaped one-wave kernels (kernels_small.hpp) for every (d, chi)
@@ -108,9 +126,26 @@ inline void launch_mfma8(ClassView cv, const cplx<float>* T, cplx<float>* mA, cp
                        mA, mB, st, rho, bloch);
 }
 
+// FP32, chi = 16, degree 2 or 3: k_mfma16_bp, eight waves per degree-3 node (128 KB of LDS,
+// one node per CU), one wave per degree-2 node.
+template <int D, int W>
+inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
+                             hipStream_t s) {
+  constexpr int LDS = Mf16Lds<D, W>::BYTES;
+  const int grid = node_grid_lds<k_mfma16_bp<D, W>, 64 * W, LDS>(cv.n);
+  hipLaunchKernelGGL((k_mfma16_bp<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, mA, mB, st);
+}
+
 template <>
 inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB,
                                   BpStatus* st, hipStream_t s) {
+  if (chi == 16 && cv.n > 0 && (cv.d == 2 || cv.d == 3)) {
+    if (cv.d == 3)
+      launch_mfma16_bp<3, 8>(cv, T, mA, mB, st, s);
+    else
+      launch_mfma16_bp<2, 1>(cv, T, mA, mB, st, s);
+    return true;
+  }
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
   launch_mfma8<kModeBP>(cv, T, mA, mB, st, nullptr, nullptr, s);

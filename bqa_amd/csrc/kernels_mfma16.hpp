@@ -1,0 +1,311 @@
+// FP32 MFMA BP sweep for bond dimension chi = 16 and degree 2 or 3 (FP32 jobs).
+//
+// One block of W waves per node.  The node tensor T[p, b_1..b_d] (complex64,
+// 64 KB at degree 3) and one work buffer live in LDS; the reference schedule
+// (backends.py:381-404) runs with the mode products in place along their bond
+// and T x_{d-1} m_{d-1} computed twice at degree 3, as in the chi = 8 kernel.
+// Complex contractions are real GEMMs on v_mfma_f32_16x16x4f32 (A[i][k] at lane
+// i + 16 k, B[k][j] at lane j + 16 k, D[4 (l / 16) + v][l % 16] in register v):
+//
+//   mode product along bond J: rows = 16 fibres, K = 32 = (component, b),
+//     N = 32 = (component, a): [Yr | Yi] = [Xr | Xi] [[m_r^T, m_i^T], [-m_i^T, m_r^T]];
+//   Gram with bond K free: K = fibres, rows (component, c) x columns (component, e),
+//     out[c][e] = G00 + G11 + i (G01 - G10) of the four 16 x 16 tiles.
+//
+// LDS layout: complex element e = ((p 16 + b_1) 16 + b_2) 16 + b_3 (degree 3)
+// is stored at sw16(e), which flips bit 0 of b_2 by bit 0 of b_1 and the four
+// bits of b_3 by b_1 ^ b_2 (degree 2: the same with (p, b_1, b_2)).  The map is
+// linear over GF(2).  With it, the 32 lanes of a half-wave touch 32 distinct
+// complex slots mod 32 in every access below: mode-product operand reads along
+// each bond, output writes (the 16 rows of a tile are ordered so that rows
+// 4 kq + v and 4 (kq + 1) + v differ in parity) and Gram operand reads.
+#pragma once
+
+#include "common.hpp"
+#include "kernels_mfma.hpp"
+#include "kernels_node.hpp"
+
+namespace bqa {
+
+constexpr int cpow16(int e) { return e <= 0 ? 1 : 16 * cpow16(e - 1); }
+
+template <int D>
+struct Mf16 {
+  static constexpr int A = cpow16(D);      // chi^d
+  static constexpr int S = 2 * A;          // complex elements per node
+  static constexpr int FIB = S / 16;       // fibres along one bond
+  static constexpr int TILES = FIB / 16;   // 16-fibre tiles of a mode product
+  static constexpr int KS = FIB / 4;       // k-steps of a Gram
+};
+
+template <int D>
+__host__ __device__ constexpr int sw16(int e) {
+  if (D == 3) {
+    const int b1 = (e >> 8) & 15, b2 = (e >> 4) & 15;
+    return e ^ ((b1 & 1) << 4) ^ ((b1 ^ b2) & 15);
+  }
+  const int p = (e >> 8) & 1, b1 = (e >> 4) & 15;
+  return e ^ ((p & 1) << 4) ^ ((p ^ b1) & 15);
+}
+
+// First element of fibre f along bond J (a zero field inserted at J).
+template <int D, int J>
+__device__ __forceinline__ int fbase16(int f) {
+  constexpr int st = cpow16(D - 1 - J);
+  return (f / st) * 16 * st + (f % st);
+}
+
+// Row r of a mode-product tile -> fibre offset within the tile: rows 4 kq + v and
+// 4 (kq + 1) + v land on fibres of different parity.
+__device__ __forceinline__ constexpr int row_perm16(int r) { return ((r & 3) << 1) | ((r >> 2) & 1) | (r & 8); }
+
+__device__ __forceinline__ f4v mfma4f(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// B operands of message m (16 x 16 complex64) for a mode product: bm[s][ct] for k-step
+// s (k = 4 s + kq) and column tile ct, from [[m_r^T, m_i^T], [-m_i^T, m_r^T]].
+__device__ __forceinline__ void msg_operand16(const cplx<float>* __restrict__ m, float (&bm)[8][2]) {
+  const int l = threadIdx.x & 63, a = l & 15, kq = l >> 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const cplx<float> v = m[a * 16 + 4 * q + kq];   // b = 4 q + kq
+    bm[q][0] = v.x;          // k < 16 (Re X): column Re Y -> m_r, column Im Y -> m_i
+    bm[q][1] = v.y;
+    bm[q + 4][0] = -v.y;     // k >= 16 (Im X): column Re Y -> -m_i, column Im Y -> m_r
+    bm[q + 4][1] = v.x;
+  }
+}
+
+__device__ __forceinline__ float2 lds_ld2(const float2* base, int byte_off) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+__device__ __forceinline__ void lds_st2(float2* base, int byte_off, float2 v) {
+  *reinterpret_cast<float2*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+// Every LDS index below is built from disjoint bit fields (fibre fields and the element
+// along the bond), and fbase16 and sw16 are linear over GF(2), so an address is
+// sw16(lane part) ^ sw16(constant part): the lane part (with the wave's offset) is formed
+// once per call and hidden from loop-invariant code motion by opaque(), the constant part
+// folds at compile time, and each access costs one XOR.
+
+// Y = X x_J m over this wave's tiles (float2 views of the swizzled LDS buffers); X may equal Y.
+template <int D, int W, int J>
+__device__ __forceinline__ void mp16(const float2* X, float2* Y, const cplx<float>* __restrict__ m) {
+  float bm[8][2];                           // loaded per product: the node's operands do not stay live
+  msg_operand16(m, bm);
+  constexpr int st = cpow16(D - 1 - J);
+  constexpr int TPW = Mf16<D>::TILES / W;
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+  const int wbase = sw16<D>(fbase16<D, J>(w * TPW * 16));
+  // A reads: fibre row_perm16(i16), element kq along J
+  const int rd = opaque(8 * (wbase ^ sw16<D>(fbase16<D, J>(row_perm16(i16)) ^ (kq * st))));
+  // D writes: element i16 along J (the row's fibre is a per-(kq, v) part below)
+  const int wr = opaque(8 * (wbase ^ sw16<D>(i16 * st)));
+  const int kqr = 4 * kq;
+  float a[TPW][8];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float2 v = lds_ld2(X, rd ^ (8 * sw16<D>(fbase16<D, J>(t * 16) ^ (4 * q * st))));
+      a[t][q] = v.x;
+      a[t][q + 4] = v.y;
+    }
+  f4v acc[TPW][2];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t][0] = acc[t][1] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      acc[t][0] = mfma4f(a[t][s], bm[s][0], acc[t][0]);
+      acc[t][1] = mfma4f(a[t][s], bm[s][1], acc[t][1]);
+    }
+  // row 4 kq + v of tile t: fibre t * 16 + row_perm16(4 kq + v); row_perm16 is linear too,
+  // row_perm16(4 kq + v) = row_perm16(4 kq) ^ row_perm16(v)
+  const int rk = opaque(8 * sw16<D>(fbase16<D, J>(row_perm16(kqr))));
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      lds_st2(Y, wr ^ rk ^ (8 * sw16<D>(fbase16<D, J>(t * 16 + row_perm16(v)))),
+              make_float2(acc[t][0][v], acc[t][1][v]));
+}
+
+// This wave's k-steps of out[c][e] = sum_f conj(X[f][c]) Y[f][e] over the fibres along bond K:
+// lane (c = 4 kq + v, e = l % 16) gets its four entries, FP32 chains combined in FP64.
+template <int D, int W, int K>
+__device__ __forceinline__ void gram16(const float2* X, const float2* Y, double (&gre)[4], double (&gim)[4]) {
+  constexpr int st = cpow16(D - 1 - K);
+  constexpr int KPW = Mf16<D>::KS / W;
+  constexpr int B = KPW < 4 ? KPW : 4;
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+  // fibre 4 (w KPW + s) + kq, element i16 along K
+  const int gl = opaque(8 * (sw16<D>(fbase16<D, K>(4 * w * KPW)) ^ sw16<D>(fbase16<D, K>(kq) ^ (i16 * st))));
+  f4v g[2][2][2];                          // [chain][row tile][column tile]
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+    g[c][0][0] = g[c][0][1] = g[c][1][0] = g[c][1][1] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s0 = 0; s0 < KPW; s0 += B) {
+    float2 xa[B], ya[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int off = gl ^ (8 * sw16<D>(fbase16<D, K>(4 * (s0 + b))));
+      xa[b] = lds_ld2(X, off);
+      ya[b] = lds_ld2(Y, off);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int c = b & 1;
+      g[c][0][0] = mfma4f(xa[b].x, ya[b].x, g[c][0][0]);
+      g[c][1][1] = mfma4f(xa[b].y, ya[b].y, g[c][1][1]);
+      g[c][0][1] = mfma4f(xa[b].x, ya[b].y, g[c][0][1]);
+      g[c][1][0] = mfma4f(xa[b].y, ya[b].x, g[c][1][0]);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    gre[v] = ((double)g[0][0][0][v] + (double)g[1][0][0][v]) + ((double)g[0][1][1][v] + (double)g[1][1][1][v]);
+    gim[v] = ((double)g[0][0][1][v] + (double)g[1][0][1][v]) - ((double)g[0][1][0][v] + (double)g[1][1][0][v]);
+  }
+}
+
+// LDS: the node tensor, the work buffer, and a finishing area (per-wave Gram partials and the
+// summed Gram in FP64) that overlays the work buffer when it is large enough.
+template <int D, int W>
+struct Mf16Lds {
+  static constexpr int TBYTES = Mf16<D>::S * 8;
+  static constexpr int FIN = W * 256 * 16 + 256 * 16;   // partials (double2) + summed Gram (double2)
+  static constexpr int BYTES = 2 * TBYTES + (FIN > TBYTES ? FIN - TBYTES : 0);
+};
+
+// Sum the W partials, keep the Hermitian part normalised by the real trace, store the
+// message at position pos and update this thread's distance maxima (squared magnitudes).
+template <int W>
+__device__ __forceinline__ void finish16(double2* part, double2* S, long pos, const cplx<float>* __restrict__ min,
+                                         cplx<float>* mout, double& dnum2, double& dden2) {
+  for (int o = threadIdx.x; o < 256; o += 64 * W) {
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      re += part[w * 256 + o].x;
+      im += part[w * 256 + o].y;
+    }
+    S[o] = make_double2(re, im);
+  }
+  __syncthreads();
+  double tr = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) tr += S[c * 17].x;
+  const double inv = 1.0 / tr;
+  for (int o = threadIdx.x; o < 256; o += 64 * W) {
+    const int c = o >> 4, e = o & 15;
+    const double2 a = S[o], b = S[e * 16 + c];
+    const cplx<float> v = mk((float)(0.5 * (a.x + b.x) * inv), (float)(0.5 * (a.y - b.y) * inv));
+    const cplx<double> vd = cast<double>(v), od = cast<double>(min[pos * 256 + o]);
+    dnum2 = fmax(dnum2, abs2(vd - od));
+    dden2 = fmax(dden2, abs2(vd + od));
+    mout[pos * 256 + o] = v;
+  }
+}
+
+template <int D, int W, int K>
+__device__ __forceinline__ void gram_finish16(const float2* T, float2* Wb, long pos, const cplx<float>* min,
+                                              cplx<float>* mout, double& dnum2, double& dden2) {
+  double gre[4], gim[4];
+  gram16<D, W, K>(T, Wb, gre, gim);
+  __syncthreads();                                      // every wave has read Wb
+  double2* part = reinterpret_cast<double2*>(Wb);
+  double2* S = part + W * 256;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, kq = l >> 4, e = l & 15;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) part[w * 256 + (4 * kq + v) * 16 + e] = make_double2(gre[v], gim[v]);
+  __syncthreads();
+  finish16<W>(part, S, pos, min, mout, dnum2, dden2);
+  __syncthreads();                                      // the next mode product overwrites Wb
+}
+
+template <int D, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma16_bp(ClassView cv, const cplx<float>* __restrict__ T,
+                                                      cplx<float>* msgA, cplx<float>* msgB, BpStatus* st) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(Mf16<D>::TILES % W == 0 && Mf16<D>::KS % W == 0, "tiles and k-steps split evenly over the waves");
+  static_assert(Mf16<D>::S % (128 * W) == 0, "tensor load in whole rounds of float4");
+  extern __shared__ __align__(16) unsigned char smem16[];
+  float2* Ts = reinterpret_cast<float2*>(smem16);
+  float2* Wb = Ts + Mf16<D>::S;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const int n = cv.n;
+  const cplx<float>* min = cur ? msgB : msgA;
+  cplx<float>* mout = cur ? msgA : msgB;
+  double dnum = 0.0, dden = 0.0;
+  constexpr int S = Mf16<D>::S;
+  constexpr int NT = 64 * W;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    // tensor: float4 q holds elements 2q and 2q + 1, whose swizzled slots are one aligned pair
+    const float4* Tg = reinterpret_cast<const float4*>(T + (size_t)i * S);
+    float4* T4 = reinterpret_cast<float4*>(Ts);
+#pragma unroll
+    for (int k = 0; k < S / (2 * NT); ++k) {
+      const int q = threadIdx.x + NT * k;
+      const float4 v = Tg[q];
+      const int sw = sw16<D>(2 * q);
+      T4[sw >> 1] = (sw & 1) ? make_float4(v.z, v.w, v.x, v.y) : v;
+    }
+    const cplx<float>* m[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) m[j] = min + (size_t)cv.in_pos[j * n + i] * 256;
+    __syncthreads();
+    if constexpr (D == 2) {
+      mp16<D, W, 1>(Ts, Wb, m[1]);                     // W = T x1 m1
+      __syncthreads();
+      gram_finish16<D, W, 0>(Ts, Wb, cv.out_pos[i], min, mout, dnum, dden);
+      mp16<D, W, 0>(Ts, Wb, m[0]);                     // W = T x0 m0
+      __syncthreads();
+      gram_finish16<D, W, 1>(Ts, Wb, cv.out_pos[n + i], min, mout, dnum, dden);
+    } else {
+      mp16<D, W, 2>(Ts, Wb, m[2]);                     // W = T x2 m2
+      __syncthreads();
+      mp16<D, W, 1>(Wb, Wb, m[1]);                     // W = W x1 m1
+      __syncthreads();
+      gram_finish16<D, W, 0>(Ts, Wb, cv.out_pos[i], min, mout, dnum, dden);
+      mp16<D, W, 2>(Ts, Wb, m[2]);                     // W = T x2 m2
+      __syncthreads();
+      mp16<D, W, 0>(Wb, Wb, m[0]);                     // W = W x0 m0
+      __syncthreads();
+      gram_finish16<D, W, 1>(Ts, Wb, cv.out_pos[n + i], min, mout, dnum, dden);
+      mp16<D, W, 0>(Ts, Wb, m[0]);                     // W = T x0 m0
+      __syncthreads();
+      mp16<D, W, 1>(Wb, Wb, m[1]);                     // W = W x1 m1
+      __syncthreads();
+      gram_finish16<D, W, 2>(Ts, Wb, cv.out_pos[2 * n + i], min, mout, dnum, dden);
+    }
+  }
+  // distance maxima of squared magnitudes: block reduction, one square root per block
+  __shared__ double red[2][W];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+    dden = fmax(dden, __shfl_xor(dden, o, 64));
+  }
+  if (l == 0) {
+    red[0][w] = dnum;
+    red[1][w] = dden;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < W; ++k) {
+      a = fmax(a, red[0][k]);
+      b = fmax(b, red[1][k]);
+    }
+    bp_end(st, cv.sweep, sqrt(a), sqrt(b));
+  }
+}
+
+}  // namespace bqa
