@@ -564,7 +564,8 @@ void launch_canon_h(bqa_job* j, int ccols, double pinv_eps, double deps) {
   LAUNCH_CHECK();
 }
 
-// Register-resident canonicalisers for chi <= 8 (split-column Cholesky + Jacobi); false for larger chi.
+// Register-resident canonicalisers for chi <= 8 and chi = 16 (split-column Cholesky + Jacobi); false
+// for the other bond dimensions.
 bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double deps) {
   switch (chi) {
     case 1: launch_canon_h<2>(j, ccols, pinv_eps, deps); return true;
@@ -575,6 +576,7 @@ bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double 
     case 6: launch_canon_h<12>(j, ccols, pinv_eps, deps); return true;
     case 7: launch_canon_h<14>(j, ccols, pinv_eps, deps); return true;
     case 8: launch_canon_h<16>(j, ccols, pinv_eps, deps); return true;
+    case 16: launch_canon_h<32>(j, ccols, pinv_eps, deps); return true;
     default: return false;
   }
 }

@@ -1,4 +1,4 @@
-// Canonicalisers with split columns for bond dimensions chi <= 8 (n = 2 chi <= 16).
+// Canonicalisers with split columns for bond dimensions chi <= 8 and chi = 16 (n = 2 chi <= 32).
 //
 // Same results as k_canonicalize, organised for occupancy and
 // fewer Jacobi sweeps:
@@ -28,7 +28,7 @@ namespace bqa {
 template <int N>
 struct HCanon {
   static constexpr int P = N / 2;                                   // column pairs
-  static constexpr int P2 = P <= 1 ? 1 : P <= 2 ? 2 : P <= 4 ? 4 : 8;
+  static constexpr int P2 = P <= 1 ? 1 : P <= 2 ? 2 : P <= 4 ? 4 : P <= 8 ? 8 : 16;
   static constexpr int G = 2 * P2;                                  // lanes per edge
   static constexpr int EPW = 64 / G;                                // edges per wave
   static constexpr int NH = N / 2;                                  // rows per lane
@@ -80,7 +80,10 @@ __device__ __forceinline__ double xor_lane(double x) {
 
 template <int N>
 __device__ __forceinline__ double half_sum(double v) {
-  return v + xor_lane<HCanon<N>::P2>(v);
+  if constexpr (HCanon<N>::P2 == 16)
+    return v + __shfl_xor(v, 16, 64);     // the partner half is in the other 16-lane row
+  else
+    return v + xor_lane<HCanon<N>::P2>(v);
 }
 
 template <int N>
@@ -335,7 +338,7 @@ __device__ void hcholesky(HCol<N / 2>& top, HCol<N / 2>& bot, int k, int h, bool
 }
 
 template <int N>
-__global__ __launch_bounds__(64, 3) void k_canon_h(const cplx<double>* __restrict__ ext, int E, int ccols,
+__global__ __launch_bounds__(64, N <= 16 ? 3 : 1) void k_canon_h(const cplx<double>* __restrict__ ext, int E, int ccols,
                                                 double pinv_eps, double deps, zd* __restrict__ canon,
                                                 double* __restrict__ lfull, double* __restrict__ ws) {
   using S = HCanon<N>;

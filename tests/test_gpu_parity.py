@@ -513,3 +513,16 @@ def test_chi16_mfma_bp_matches_generic(name):
     finally:
         for job in jobs:
             job.close()
+    # the simple update from the same state: the n = 32 split-column canonicaliser against the generic
+    # one (FP64 jobs, so only the canonicaliser's arithmetic differs), chi' and sorted lambdas
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, "double", g) for g in (False, True)]
+    try:
+        su = [job.simple_update(layers[k]["ztime"], ctx_o.pinv_eps) for job in jobs]
+        lm = [np.sort(job.get_lambdas().real, axis=1) for job in jobs]
+        print(f"[gpu]   simple update chi' {su[0][0]} / {su[1][0]}, |lambda fast - generic| "
+              f"{np.abs(lm[0] - lm[1]).max():.2e}", flush=True)
+        assert su[0][0] == su[1][0]
+        assert np.abs(lm[0] - lm[1]).max() < 1e-10
+    finally:
+        for job in jobs:
+            job.close()
