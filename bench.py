@@ -58,9 +58,14 @@ BP_EPS = 1e-6
 BP_ITERS = 75
 PINV_EPS = 1e-6
 SERVE_SEED = 5
+SERVE_SLOTS = 16             # resident jobs of the batched serving executor
 SAMPLE_QUBITS = 100          # oracle sample: full 200-step anneal on the CPU (~30 s, one core)
 GRID_SIDE = 100
 GRID_STEPS = 10              # timed chi = 8 steps of the 100 x 100 grid line
+HHEX_CELLS = 64              # heavy-hex lattice of 64 x 64 cells: 10,144 qubits, degrees 2 and 3
+HHEX_CHI = 16
+HHEX_STEPS = 300
+HHEX_TIMED = 10
 TAG = {"single": "fp32", "double": "fp64"}
 
 
@@ -86,6 +91,15 @@ def grid_config(side, chi, steps, precision):
             "precision": precision}
 
 
+def hhex_config(chi, steps, precision):
+    from bqa_amd import instances
+    nodes, edges = instances.heavy_hex_gaussian(HHEX_CELLS, HHEX_CELLS, seed=4, field_sigma=0.5)
+    return {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "bp_eps": BP_EPS,
+            "max_bp_iter_number": BP_ITERS, "pinv_eps": PINV_EPS, "damping": 0.0,
+            "schedule": instances.anneal_schedule(30.0, steps), "backend": "mi355x",
+            "precision": precision}
+
+
 def run_anneal(ctx, precision, warmup, steps, finish=True, timed=True):
     """Run one anneal on the GPU.
 
@@ -108,6 +122,9 @@ def run_anneal(ctx, precision, warmup, steps, finish=True, timed=True):
                                 ctx.bp_eps, ctx.damping)
             step_s.append(time.perf_counter() - t)
             stats.append(out)
+            if (k + 1) % 50 == 0:
+                log(f"[bench]   step {k + 1}/{len(layers)}: chi {out['bond_dim']}, {out['sweeps']} sweeps, "
+                    f"{1e3 * step_s[-1]:.1f} ms")
 
         t_all = time.perf_counter()
         k = 0
@@ -263,6 +280,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chi4", action="store_true")
     ap.add_argument("--no-grid", action="store_true")
+    ap.add_argument("--no-hhex", action="store_true", help="skip the heavy-hex chi=16 line")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the serving sample (0 skips it)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="also run the plain copy kernel on BYTES bytes (HBM counter calibration)")
@@ -347,32 +365,65 @@ def main():
                          f"200-step schedule, chi={args.chi}; {GRID_STEPS} steps timed once the bond dimension is "
                          f"{args.chi}")
         extra["grid100"] = g
+    if not args.no_hhex:
+        hx, runs = {}, {}
+        nq = None
+        for prec in ("single", "double"):
+            ctx = config_to_context(hhex_config(HHEX_CHI, HHEX_STEPS, prec))
+            nq = ctx.nodes_number
+            log(f"[bench] heavy-hex {nq} qubits chi={HHEX_CHI} {TAG[prec]}: {HHEX_STEPS}-step anneal, "
+                f"{HHEX_TIMED} steps timed at chi={HHEX_CHI}")
+            r = run_anneal(ctx, prec, args.warmup, HHEX_TIMED, finish=not args.no_accuracy)
+            hs = summarize(r, nq, HHEX_TIMED)
+            kern = ("k_mfma16_bp<3,8> / <2,1> (FP32 MFMA v_mfma_f32_16x16x4f32)" if prec == "single"
+                    else "k_node_contract<double> (generic FP64)")
+            hs["roofline"] = roofline_of(hs, ctx, HHEX_CHI, prec, f"BP sweep at chi={HHEX_CHI}, d=2,3: {kern}")
+            hx[TAG[prec]] = hs
+            runs[prec] = r
+            log(f"[bench] heavy-hex {TAG[prec]}: {hs['qubit_steps_per_s']:.4g} qubit*steps/s, "
+                f"{hs['ms_per_step']:.2f} ms/step, BP MFMA frac {hs['roofline']['mfma_frac']:.3f}; full anneal "
+                f"{hs['full_anneal']['seconds']:.1f} s")
+        if not args.no_accuracy:
+            hx["fp32_vs_fp64_gpu"] = accuracy(runs["single"], runs["double"])
+            log(f"[bench] heavy-hex fp32 vs fp64: {hx['fp32_vs_fp64_gpu']}")
+        hx["workload"] = (f"heavy-hex lattice of {HHEX_CELLS}x{HHEX_CELLS} cells ({nq} qubits, degrees 2/3), "
+                          f"J ~ N(0,1), h ~ N(0,0.5), seed 4, T=30, {HHEX_STEPS} Trotter steps, chi={HHEX_CHI}; "
+                          f"{HHEX_TIMED} steps timed once the bond dimension is {HHEX_CHI}; fp32 = complex64 "
+                          "tensors/messages on FP32 MFMA with FP64 canonicalisers, fp64 = complex128 throughout")
+        extra["heavy_hex_chi16"] = hx
     serving = None
     if args.serve_jobs > 0:
         from bqa_amd.serve import mixed_requests, serve
         log(f"[bench] serving: {args.serve_jobs} mixed jobs, Poisson arrivals")
         reqs = mixed_requests(args.serve_jobs, seed=SERVE_SEED, n_min=100, n_max=5000, rate_per_s=8.0,
-                              degrees=(3,), chis=(4, 8))
-        sv = serve(reqs, max_concurrent=8)
+                              degrees=(3, 4, 5), chis=(4, 8), chi_cap={5: 4})
+        sv = serve(reqs, max_concurrent=SERVE_SLOTS)
         keep = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "outcomes"}
+        pick = ("jobs", "failed", "makespan_s", "jobs_per_h", "qubit_steps_per_s", "p50_s", "p99_s")
         serving = keep(sv)
-        serving["workload"] = (f"{args.serve_jobs} jobs, Poisson arrivals at 8/s (seed {SERVE_SEED}); random 3-regular, "
-                               "n log-uniform in [100, 5000], chi in {4, 8}, J ~ U[-1,1], h = 0, T = 10, "
-                               "100-300 Trotter steps growing with log n; FP32; at most 8 resident jobs")
+        serving["workload"] = (f"{args.serve_jobs} jobs, Poisson arrivals at 8/s (seed {SERVE_SEED}); random "
+                               "d-regular with d in {3, 4, 5}, n log-uniform in [100, 5000], chi in {4, 8} (chi 4 at "
+                               "d = 5), J ~ U[-1,1], h = 0, T = 10, 100-300 Trotter steps growing with log n; FP32; "
+                               f"at most {SERVE_SLOTS} resident jobs, batched executor (one bqa_batch step of all "
+                               "resident jobs per round, BP sweeps in launches shared per shape)")
         serving["note"] = ("jobs_per_h and p50/p99 at the offered load above; capacity: the same jobs all queued "
-                           "at t = 0")
+                           "at t = 0; one_at_a_time: each job alone on its own bqa_job (no overlap, no sharing); "
+                           "threaded: the same queue on one host thread and stream per job")
+        serving["mix"] = {f"d{d}_chi{c}": sum(1 for r in reqs if r.tag["degree"] == d and r.tag["chi"] == c)
+                          for d in (3, 4, 5) for c in (4, 8)}
         for r in reqs:
             r.arrival = 0.0
-        cap = serve(reqs, max_concurrent=8)
-        serving["capacity"] = {k: v for k, v in keep(cap).items() if k in ("jobs", "failed", "makespan_s", "jobs_per_h",
-                                                                        "qubit_steps_per_s", "p50_s", "p99_s")}
-        seq = serve(reqs, max_concurrent=1)
-        serving["one_at_a_time"] = {k: v for k, v in keep(seq).items() if k in ("jobs", "failed", "makespan_s",
-                                                                               "jobs_per_h", "p50_s", "p99_s")}
+        cap = serve(reqs, max_concurrent=SERVE_SLOTS)
+        serving["capacity"] = {k: v for k, v in keep(cap).items() if k in pick}
+        thr = serve(reqs, max_concurrent=8, batched=False)
+        serving["threaded"] = {k: v for k, v in keep(thr).items() if k in pick}
+        seq = serve(reqs, max_concurrent=1, batched=False)
+        serving["one_at_a_time"] = {k: v for k, v in keep(seq).items() if k in pick}
         serving["capacity_over_one_at_a_time"] = round(cap["jobs_per_h"] / max(1e-12, seq["jobs_per_h"]), 3)
         log(f"[bench] serving: {serving['jobs_per_h']:.4g} jobs/h offered, p50 {serving['p50_s']:.3f} s, "
-            f"p99 {serving['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h; failed {serving['failed']}; "
-            f"one at a time {seq['jobs_per_h']:.4g} jobs/h ({serving['capacity_over_one_at_a_time']:.2f}x)")
+            f"p99 {serving['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h (threaded "
+            f"{thr['jobs_per_h']:.4g}); failed {serving['failed']}; one at a time {seq['jobs_per_h']:.4g} jobs/h "
+            f"({serving['capacity_over_one_at_a_time']:.2f}x)")
     cpu, oracle_parity = None, None
     if not args.no_cpu:
         log(f"[bench] cpu baseline and oracle parity sample ({SAMPLE_QUBITS} qubits, {BASE_STEPS} steps)")

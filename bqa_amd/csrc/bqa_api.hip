@@ -13,10 +13,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/bqa_hip.h"
@@ -69,6 +71,16 @@ int guarded(F&& f) {
   }
 }
 
+// Runtime calls that must not overlap another thread's stream capture (BP chunk
+// graphs): device and pinned allocation, their release, and stream creation and
+// destruction, which synchronise beyond one stream.  Captures hold the same lock.
+// No call of this library uses the legacy NULL stream (bqa_energy_noalloc runs on
+// the stream its caller passes): while any stream is being captured, HIP fails a
+// legacy-stream operation in another thread ("would make the legacy stream depend
+// on a capturing blocking stream") and invalidates the capture, so uploads and
+// copies go through the job's own stream.
+std::mutex g_capture_mu;
+
 // Device buffer that only grows.
 struct DevBuf {
   void* p = nullptr;
@@ -76,6 +88,7 @@ struct DevBuf {
   void ensure(size_t b, size_t& total, hipStream_t s = nullptr) {
     if (b <= bytes) return;
     if (s) HIPX(hipStreamSynchronize(s));
+    std::lock_guard<std::mutex> g(g_capture_mu);
     if (p) {
       HIPX(hipFree(p));
       total -= bytes;
@@ -88,7 +101,10 @@ struct DevBuf {
     total += b;
   }
   void release(size_t& total) {
-    if (p) (void)hipFree(p);
+    if (p) {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      (void)hipFree(p);
+    }
     total -= bytes;
     p = nullptr;
     bytes = 0;
@@ -180,6 +196,22 @@ struct bqa_job {
   // c (canonicaliser), l (absorb/gates/gauge) selects them per family.
   unsigned generic_mask = 0;
   bool generic(unsigned fam) const { return (generic_mask & fam) != 0; }
+  // BP launch variants: BQA_AUTOTUNE=0 keeps variant 0 everywhere; BQA_BP_VARIANT=k runs
+  // variant k wherever the shape has one (both read at job creation).
+  bool autotune = true;
+  int bp_variant_force = -1;
+  // Graph-captured BP chunks (BQA_GRAPHS=0 turns them off): kGraphSweeps sweeps of every class
+  // plus the sweep-index advance, captured once per key (bond dimension, buffers, variants)
+  // and replayed once the job has run BP kGraphWarmRuns times in a row at its bond dimension.
+  struct BpGraph {
+    std::vector<intptr_t> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  bool use_graphs = true;
+  int graph_chi = 0;   // bond dimension of the last completed BP runs
+  int graph_runs = 0;  // consecutive completed BP runs at graph_chi
+  std::vector<BpGraph> graphs;
+  int64_t graph_captures = 0, graph_launches = 0, graph_failures = 0;
 
   bool single() const { return prec == BQA_PREC_SINGLE; }
 };
@@ -233,9 +265,14 @@ void ensure_msgs(bqa_job* j, int chi) {
     for (int w = 0; w < 2; ++w) {
       if (b <= j->msgs[w].bytes) continue;
       void* np = nullptr;
-      HIPX(hipMalloc(&np, b));
+      {
+        std::lock_guard<std::mutex> g(g_capture_mu);
+        HIPX(hipMalloc(&np, b));
+      }
       if (j->msgs[w].p) {
-        HIPX(hipMemcpy(np, j->msgs[w].p, j->msgs[w].bytes, hipMemcpyDeviceToDevice));
+        HIPX(hipMemcpyAsync(np, j->msgs[w].p, j->msgs[w].bytes, hipMemcpyDeviceToDevice, j->stream));
+        HIPX(hipStreamSynchronize(j->stream));
+        std::lock_guard<std::mutex> g(g_capture_mu);
         HIPX(hipFree(j->msgs[w].p));
         j->dev_bytes -= j->msgs[w].bytes;
       }
@@ -304,6 +341,7 @@ __global__ void k_bp_reset(BpStatus* st, int cur, int max_iters, double bp_eps) 
   st->damp_now = 0;
   st->max_iters = max_iters;
   st->marker = 0;
+  st->base = 0;
   for (int k = 0; k < kSweepSlots && k < max_iters; ++k) {
     st->slot[k][0] = 0ULL;
     st->slot[k][1] = 0ULL;
@@ -312,6 +350,10 @@ __global__ void k_bp_reset(BpStatus* st, int cur, int max_iters, double bp_eps) 
   st->last_dist = 0.0;
   st->best_dist = INFINITY;
 }
+
+// End of a graph-replayed chunk of sweeps: the next replay's launches take the
+// following sweep indices.
+__global__ void k_bp_advance(BpStatus* st, int by) { st->base += by; }
 
 // After every sweep: distance, convergence and buffer bookkeeping
 // (src/bqa/state.py:113-121).
@@ -437,21 +479,103 @@ int grid_for(long n, int block = 256, int cap = 8192) {
 
 // ---------------------------------------------------------------- phases --
 
-// One class's launch of a BP sweep; sweep >= 0 selects the sweep-slot protocol.
+// ---------------------------------------------------------- autotuner --
+//
+// Process-wide choice of BP launch variant (kernels_fast.hpp) per device,
+// precision, degree, chi and ceil(log2 n).  A shape is timed once, on the first
+// sweep of a BP run that meets it: every variant runs that sweep twice.  Each
+// launch reads the sweep's input buffer and writes the same outputs and the same
+// distance maxima, so the repeats leave the run exactly as one launch would;
+// the variants differ only in time (grid policies: bit for bit; wave splits:
+// to rounding).  While one job times a shape, others use variant 0.
+struct TuneKey {
+  int dev, prec, d, chi, nlog;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dev, prec, d, chi, nlog) < std::tie(o.dev, o.prec, o.d, o.chi, o.nlog);
+  }
+};
+struct TuneEntry {
+  int chosen = -1;  // -1 while being timed
+  std::vector<double> us;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+constexpr int kTuneReps = 2;
+
+int ceil_log2(long n) {
+  int k = 0;
+  while ((1L << k) < n) ++k;
+  return k;
+}
+
 template <typename R>
-void launch_bp_class(bqa_job* j, ClassDev& c, int sweep) {
+int bp_variant(bqa_job* j, ClassDev& c, const ClassView& view, bool may_tune) {
+  const int chi = j->chi;
+  const int nv = bp_variant_count<R>(c.d, chi);
+  if (nv <= 1) return 0;
+  if (j->bp_variant_force >= 0) return j->bp_variant_force < nv ? j->bp_variant_force : 0;
+  if (!j->autotune) return 0;
+  const TuneKey key{j->device, j->prec, c.d, chi, ceil_log2(c.n)};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second.chosen >= 0 ? it->second.chosen : 0;
+    if (!may_tune) return 0;
+    g_tune[key];
+  }
+  const cplx<R>* T = reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p);
+  BpStatus* st = reinterpret_cast<BpStatus*>(j->status.p);
+  std::vector<hipEvent_t> ev(nv + 1, nullptr);
+  TuneEntry res;
+  try {
+    for (auto& e : ev) HIPX(hipEventCreate(&e));
+    fast_bp_launch<R>(view, chi, T, msg_ptr<R>(j, 0), msg_ptr<R>(j, 1), st, j->stream, 0);  // warm-up
+    LAUNCH_CHECK();
+    for (int v = 0; v < nv; ++v) {
+      HIPX(hipEventRecord(ev[v], j->stream));
+      for (int r = 0; r < kTuneReps; ++r) {
+        fast_bp_launch<R>(view, chi, T, msg_ptr<R>(j, 0), msg_ptr<R>(j, 1), st, j->stream, v);
+        LAUNCH_CHECK();
+      }
+    }
+    HIPX(hipEventRecord(ev[nv], j->stream));
+    HIPX(hipEventSynchronize(ev[nv]));
+    res.us.resize(nv);
+    res.chosen = 0;
+    for (int v = 0; v < nv; ++v) {
+      float ms = 0.f;
+      HIPX(hipEventElapsedTime(&ms, ev[v], ev[v + 1]));
+      res.us[v] = 1e3 * ms / kTuneReps;
+      if (res.us[v] < res.us[res.chosen]) res.chosen = v;
+    }
+  } catch (...) {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto e : ev) (void)hipEventDestroy(e);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = res;
+  return res.chosen;
+}
+
+// One class's launch of a BP sweep; sweep >= 0 selects the sweep-slot protocol.
+// `first` marks the run's first sweep, where the autotuner may time the shape.
+template <typename R>
+void launch_bp_class(bqa_job* j, ClassDev& c, int sweep, bool first) {
   const int chi = j->chi;
   if (c.d == 0 || c.n == 0) return;
+  ClassView view = c.view();
+  view.sweep = sweep;
+  const int variant = j->generic(kGenBP) ? 0 : bp_variant<R>(j, c, view, first);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (j->timing) {
     e0 = take_event(j);
     e1 = take_event(j);
     HIPX(hipEventRecord(e0, j->stream));
   }
-  ClassView view = c.view();
-  view.sweep = sweep;
   if (j->generic(kGenBP) || !fast_bp_launch<R>(view, chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
-                         msg_ptr<R>(j, 1), reinterpret_cast<BpStatus*>(j->status.p), j->stream)) {
+                         msg_ptr<R>(j, 1), reinterpret_cast<BpStatus*>(j->status.p), j->stream, variant)) {
     const long S = 2 * ipow_l(chi, c.d);
     const size_t fixed = sizeof(cplx<R>) * (size_t)(kNT + 4 * chi * chi);
     NodeLaunch L = plan_node(j, c.n, sizeof(cplx<R>) * 3 * S, fixed, 3 * S, sizeof(cplx<R>));
@@ -465,6 +589,103 @@ void launch_bp_class(bqa_job* j, ClassDev& c, int sweep) {
     HIPX(hipEventRecord(e1, j->stream));
     j->ev_pending.emplace_back(e0, e1);
     j->pending_bytes.push_back(sweep_bytes(j, c, chi));
+  }
+}
+
+// ------------------------------------------------------- graph chunks --
+
+constexpr int kGraphSweeps = 4;
+constexpr size_t kMaxGraphs = 8;
+// Eager BP runs at one bond dimension before its chunks are captured: a capture and
+// instantiation cost about as much as the launches of tens of sweeps, so graphs
+// serve the steady state, not the steps of the bond-dimension ramp.
+constexpr int kGraphWarmRuns = 8;
+
+// The chunk graph for the current buffers and launch variants, captured on first use.  Capture
+// uses the thread-local mode: other threads (other jobs) keep calling the runtime meanwhile,
+// which the global mode would refuse, failing their calls.
+template <typename R>
+hipGraphExec_t bp_graph(bqa_job* j, const std::vector<int>& variants) {
+  const int chi = j->chi;
+  std::vector<intptr_t> key = {chi, (intptr_t)j->msgs[0].p, (intptr_t)j->msgs[1].p, (intptr_t)j->status.p};
+  for (size_t ci = 0; ci < j->cls.size(); ++ci) {
+    const ClassDev& c = j->cls[ci];
+    key.push_back((intptr_t)c.T[c.tcur].p);
+    key.push_back(c.n);
+    key.push_back(variants[ci]);
+  }
+  for (const auto& g : j->graphs)
+    if (g.key == key) return g.exec;
+  BpStatus* st = reinterpret_cast<BpStatus*>(j->status.p);
+  hipGraph_t graph = nullptr;
+  std::unique_lock<std::mutex> lock(g_capture_mu);
+  HIPX(hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int s = 0; s < kGraphSweeps; ++s)
+      for (size_t ci = 0; ci < j->cls.size(); ++ci) {
+        ClassDev& c = j->cls[ci];
+        if (c.d == 0 || c.n == 0) continue;
+        ClassView view = c.view();
+        view.sweep = s;
+        if (!fast_bp_launch<R>(view, chi, reinterpret_cast<const cplx<R>*>(c.T[c.tcur].p), msg_ptr<R>(j, 0),
+                               msg_ptr<R>(j, 1), st, j->stream, variants[ci]))
+          throw BqaError(BQA_ERR_HIP, "no specialised BP kernel for a captured class");
+        LAUNCH_CHECK();
+      }
+    hipLaunchKernelGGL(k_bp_advance, dim3(1), dim3(1), 0, j->stream, st, kGraphSweeps);
+    LAUNCH_CHECK();
+  } catch (...) {
+    hipGraph_t dead = nullptr;
+    (void)hipStreamEndCapture(j->stream, &dead);
+    if (dead) (void)hipGraphDestroy(dead);
+    throw;
+  }
+  const hipError_t ec = hipStreamEndCapture(j->stream, &graph);
+  lock.unlock();
+  if (ec != hipSuccess || graph == nullptr) {
+    // a runtime call from outside this library invalidated the capture: the job
+    // goes on with eager launches
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    j->use_graphs = false;
+    ++j->graph_failures;
+    return nullptr;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPX(e);
+  if (j->graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(j->graphs.front().exec);
+    j->graphs.erase(j->graphs.begin());
+  }
+  j->graphs.push_back({std::move(key), exec});
+  ++j->graph_captures;
+  return exec;
+}
+
+// The rules of k_bp_finish (no damping, src/bqa/state.py:113-121) applied on the host to the
+// sweep slots [ruled, launched) of a run, stopping at the first that ends it.
+void apply_slot_rules(const BpStatus* sh, BpStatus& h, int& ruled, int launched, int max_iters, double bp_eps) {
+  for (; ruled < launched && !h.done; ++ruled) {
+    const double num = bits_to_double(sh->slot[ruled][0]);
+    const double den = bits_to_double(sh->slot[ruled][1]);
+    const double dist = num / den;
+    h.iters += 1;
+    h.last_dist = dist;
+    if (dist < h.best_dist) {
+      h.best_dist = dist;
+      h.have_best = 1;
+    }
+    if (dist < bp_eps) {
+      h.converged = 1;
+      h.done = 1;  // keep the previous iterate
+    } else if (h.iters >= max_iters) {
+      h.done = 1;
+      h.cur = 1 - h.cur;  // the last undamped iterate
+    } else {
+      h.cur = 1 - h.cur;
+    }
   }
 }
 
@@ -489,44 +710,49 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
   h.best_dist = INFINITY;
   int launched = 0, ruled = 0;
   int chunk = std::max(1, std::min(j->pred_sweeps, max_iters));
+  // Graph replays once this bond dimension has had an eager run (its variants chosen and the
+  // kernels' launch shapes computed), with kernel timing off and a specialised kernel per class.
+  bool graph = slots && j->use_graphs && !j->timing && !j->generic(kGenBP) && j->graph_chi == j->chi &&
+               j->graph_runs >= kGraphWarmRuns;
+  for (auto& cl : j->cls)
+    if (cl.d > 0 && cl.n > 0 && !bp_fast_shape<R>(cl.d, j->chi)) graph = false;
+  std::vector<int> variants;
+  if (graph) {
+    for (auto& cl : j->cls) {
+      ClassView view = cl.view();
+      variants.push_back(cl.d > 0 && cl.n > 0 ? bp_variant<R>(j, cl, view, false) : 0);
+    }
+  }
   while (launched < max_iters) {
-    const int c = std::min(chunk, max_iters - launched);
-    for (int s = 0; s < c; ++s) {
-      for (auto& cl : j->cls) launch_bp_class<R>(j, cl, slots ? launched + s : -1);
-      if (slots) continue;
-      hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, j->stream, st, damping_on);
-      LAUNCH_CHECK();
-      if (damping_on) {
-        hipLaunchKernelGGL((k_bp_damp<R>), dim3(grid_for(count)), dim3(256), 0, j->stream, st, msg_ptr<R>(j, 0),
-                           msg_ptr<R>(j, 1), count, damping);
+    int c = std::min(chunk, max_iters - launched);
+    const hipGraphExec_t exec = graph ? bp_graph<R>(j, variants) : nullptr;
+    if (graph && !exec) graph = false;
+    if (graph) {
+      // whole replays: their sweeps beyond max_iters, or after convergence, exit at once
+      const int reps = (c + kGraphSweeps - 1) / kGraphSweeps;
+      for (int r = 0; r < reps; ++r) {
+        HIPX(hipGraphLaunch(exec, j->stream));
+        ++j->graph_launches;
+      }
+      c = reps * kGraphSweeps;
+    } else {
+      for (int s = 0; s < c; ++s) {
+        for (auto& cl : j->cls) launch_bp_class<R>(j, cl, slots ? launched + s : -1, launched + s == 0);
+        if (slots) continue;
+        hipLaunchKernelGGL(k_bp_finish, dim3(1), dim3(1), 0, j->stream, st, damping_on);
         LAUNCH_CHECK();
+        if (damping_on) {
+          hipLaunchKernelGGL((k_bp_damp<R>), dim3(grid_for(count)), dim3(256), 0, j->stream, st, msg_ptr<R>(j, 0),
+                             msg_ptr<R>(j, 1), count, damping);
+          LAUNCH_CHECK();
+        }
       }
     }
     launched += c;
     HIPX(hipMemcpyAsync(j->st_host, st, sizeof(BpStatus), hipMemcpyDeviceToHost, j->stream));
     HIPX(hipStreamSynchronize(j->stream));
     if (slots) {
-      // the rules of k_bp_finish (no damping) over the sweeps of this chunk
-      for (; ruled < launched && !h.done; ++ruled) {
-        const double num = bits_to_double(j->st_host->slot[ruled][0]);
-        const double den = bits_to_double(j->st_host->slot[ruled][1]);
-        const double dist = num / den;
-        h.iters += 1;
-        h.last_dist = dist;
-        if (dist < h.best_dist) {
-          h.best_dist = dist;
-          h.have_best = 1;
-        }
-        if (dist < bp_eps) {
-          h.converged = 1;
-          h.done = 1;  // keep the previous iterate
-        } else if (h.iters >= max_iters) {
-          h.done = 1;
-          h.cur = 1 - h.cur;  // the last undamped iterate
-        } else {
-          h.cur = 1 - h.cur;
-        }
-      }
+      apply_slot_rules(j->st_host, h, ruled, launched, max_iters, bp_eps);
     } else {
       h = *j->st_host;
     }
@@ -545,6 +771,10 @@ void run_bp_impl(bqa_job* j, int max_iters, double bp_eps, double damping, int* 
     flush_timing(j);
   }
   j->mcur = h.cur;
+  if (slots) {
+    j->graph_runs = j->graph_chi == j->chi ? j->graph_runs + 1 : 1;
+    j->graph_chi = j->chi;
+  }
   j->pred_sweeps = std::max(1, h.iters);
   *sweeps = h.iters;
   *converged = h.converged;
@@ -618,10 +848,10 @@ void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64
   LAUNCH_CHECK();
 }
 
-void canonicalise(bqa_job* j, double pinv_eps);
+void canonicalise(bqa_job* j, double pinv_eps, bool sync = true);
 
 template <typename R>
-void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
+void ext_and_canon(bqa_job* j, double ztime, double pinv_eps, bool sync = true) {
   const int chi = j->chi, n2 = 2 * chi;
   j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
   // the step's ZZ-split gates per edge, read by the extended-message and absorb kernels
@@ -653,12 +883,13 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps) {
     if (c.d == 0 || c.n == 0) continue;
     ext_class<R>(j, c, ztime, msgs64);
   }
-  canonicalise(j, pinv_eps);
+  canonicalise(j, pinv_eps, sync);
 }
 
 // FP64 canonicalisers and lambdas of the 2chi x 2chi messages in j->ext (state.py:171-200), then
-// the per-column lambda maxima on the host (the global truncation rule reads them).
-void canonicalise(bqa_job* j, double pinv_eps) {
+// the per-column lambda maxima on the host (the global truncation rule reads them); without
+// `sync` the caller synchronises the stream before reading them.
+void canonicalise(bqa_job* j, double pinv_eps, bool sync) {
   const int chi = j->chi, n2 = 2 * chi;
   const int ccols = std::min(n2, j->chimax);
   j->canon.ensure((size_t)j->M * n2 * ccols * sizeof(zd), j->dev_bytes, j->stream);
@@ -691,7 +922,7 @@ void canonicalise(bqa_job* j, double pinv_eps) {
                      reinterpret_cast<const double*>(j->lmbd.p), j->E, n2, lm);
   LAUNCH_CHECK();
   HIPX(hipMemcpyAsync(j->lmax_host, lm, sizeof(double) * n2, hipMemcpyDeviceToHost, j->stream));
-  HIPX(hipStreamSynchronize(j->stream));
+  if (sync) HIPX(hipStreamSynchronize(j->stream));
 }
 
 // Global truncation rule (backends.py:297-303).
@@ -1079,6 +1310,180 @@ void sv_passes(bqa_sv* s, const double* ez, double ztime, double xtime, F&& each
 
 }  // namespace
 
+struct bqa_batch {
+  int device = 0;
+  int max_jobs = 0;
+  hipStream_t stream = nullptr;
+  DevBuf tab;                 // the BP launches' entry tables
+  BpEntry* tab_host = nullptr;  // pinned staging of the tables
+  size_t tab_cap = 0;         // entries both hold
+  size_t dev_bytes = 0;
+};
+
+namespace {
+
+// BP of the batch's jobs (src/bqa/state.py:97-124 per job): every job's run starts together on the
+// batch stream; each sweep is one shared launch per (precision, kernel family, degree, chi) over the
+// classes of all jobs that have it, plus the jobs' own launches for the other classes; after each
+// chunk of sweeps one copy of every running job's status and one synchronisation, then each job's
+// rules.  Jobs with damping, or more iterations than sweep slots, run their own BP.
+void batch_bp(bqa_batch* b, int count, bqa_job* const* jobs, const int32_t* max_iters, const double* bp_eps,
+              const double* damping, int32_t* sweeps, int32_t* converged, double* last_dist, int32_t* status) {
+  struct Run {
+    bqa_job* j;
+    int k;
+    BpStatus h;
+    int ruled;
+  };
+  std::vector<Run> runs;
+  for (int k = 0; k < count; ++k) {
+    bqa_job* j = jobs[k];
+    require(max_iters[k] > 0, "max_bp_iter_number must be positive: BP ran no sweep");
+    int active = 0;
+    for (auto& cl : j->cls) active += (cl.d > 0 && cl.n > 0) ? 1 : 0;
+    if (damping[k] > 0.0 || max_iters[k] > kSweepSlots || active == 0) {
+      dispatch(j, [&](auto r) {
+        int s = 0, c = 0;
+        double d = 0.0;
+        try {
+          run_bp_impl<decltype(r)>(j, max_iters[k], bp_eps[k], damping[k], &s, &c, &d);
+        } catch (const BqaError& e) {
+          if (e.code != BQA_ERR_NUMERIC) throw;
+          status[k] = BQA_ERR_NUMERIC;
+        }
+        sweeps[k] = s;
+        converged[k] = c;
+        last_dist[k] = d;
+      });
+      continue;
+    }
+    Run r{j, k, BpStatus{}, 0};
+    r.h.cur = j->mcur;
+    r.h.best_dist = INFINITY;
+    hipLaunchKernelGGL(k_bp_reset, dim3(1), dim3(1), 0, b->stream, reinterpret_cast<BpStatus*>(j->status.p),
+                       j->mcur, max_iters[k], bp_eps[k]);
+    LAUNCH_CHECK();
+    runs.push_back(r);
+  }
+  if (runs.empty()) return;
+  struct Group {
+    int prec, fam, d, chi;
+    std::vector<std::pair<int, int>> members;  // (run, class)
+    long total = 0;
+    size_t off = 0;
+  };
+  std::vector<Group> groups;
+  std::vector<std::pair<int, int>> solo;
+  for (int ri = 0; ri < (int)runs.size(); ++ri) {
+    bqa_job* j = runs[ri].j;
+    for (int ci = 0; ci < (int)j->cls.size(); ++ci) {
+      const ClassDev& cl = j->cls[ci];
+      if (cl.d == 0 || cl.n == 0) continue;
+      const int fam = j->generic(kGenBP) ? 0 : j->single() ? multi_family<float>(cl.d, j->chi)
+                                                          : multi_family<double>(cl.d, j->chi);
+      if (fam == 0) {
+        solo.emplace_back(ri, ci);
+        continue;
+      }
+      Group* g = nullptr;
+      for (auto& x : groups)
+        if (x.prec == j->prec && x.fam == fam && x.d == cl.d && x.chi == j->chi) g = &x;
+      if (!g) {
+        groups.push_back(Group{j->prec, fam, cl.d, j->chi, {}, 0, 0});
+        g = &groups.back();
+      }
+      g->members.emplace_back(ri, ci);
+    }
+  }
+  size_t entries = 0;
+  for (auto& g : groups) {
+    g.off = entries;
+    entries += g.members.size();
+  }
+  if (entries > 0) {
+    if (entries > b->tab_cap) {
+      HIPX(hipStreamSynchronize(b->stream));
+      std::lock_guard<std::mutex> lk(g_capture_mu);
+      if (b->tab_host) (void)hipHostFree(b->tab_host);
+      b->tab_host = nullptr;
+      b->tab_cap = 0;
+      HIPX(hipHostMalloc(reinterpret_cast<void**>(&b->tab_host), entries * sizeof(BpEntry), hipHostMallocDefault));
+      b->tab_cap = entries;
+    }
+    b->tab.ensure(entries * sizeof(BpEntry), b->dev_bytes, b->stream);
+    for (auto& g : groups) {
+      long prefix = 0;
+      for (size_t m = 0; m < g.members.size(); ++m) {
+        bqa_job* j = runs[g.members[m].first].j;
+        const ClassDev& cl = j->cls[g.members[m].second];
+        BpEntry& e = b->tab_host[g.off + m];
+        e.cv = cl.view();
+        e.T = cl.T[cl.tcur].p;
+        e.mA = j->msgs[0].p;
+        e.mB = j->msgs[1].p;
+        e.st = reinterpret_cast<BpStatus*>(j->status.p);
+        e.prefix = prefix;
+        prefix += cl.n;
+      }
+      g.total = prefix;
+      require(g.total <= std::numeric_limits<int>::max(), "too many nodes in one shared launch");
+    }
+    HIPX(hipMemcpyAsync(b->tab.p, b->tab_host, entries * sizeof(BpEntry), hipMemcpyHostToDevice, b->stream));
+  }
+  const BpEntry* tab = reinterpret_cast<const BpEntry*>(b->tab.p);
+  int maxit = 1, chunk = 1;
+  for (auto& r : runs) {
+    maxit = std::max(maxit, (int)max_iters[r.k]);
+    chunk = std::max(chunk, r.j->pred_sweeps);
+  }
+  chunk = std::min(chunk, maxit);
+  int launched = 0;
+  while (launched < maxit) {
+    const int c = std::min(chunk, maxit - launched);
+    for (int s = 0; s < c; ++s) {
+      const int sweep = launched + s;
+      for (auto& g : groups) {
+        const bool ok = g.prec == BQA_PREC_SINGLE
+                            ? launch_bp_multi<float>(g.fam, g.d, g.chi, tab + g.off, (int)g.members.size(),
+                                                     (int)g.total, sweep, b->stream)
+                            : launch_bp_multi<double>(g.fam, g.d, g.chi, tab + g.off, (int)g.members.size(),
+                                                      (int)g.total, sweep, b->stream);
+        if (!ok) throw BqaError(BQA_ERR_HIP, "no shared BP kernel for a batched class");
+        LAUNCH_CHECK();
+      }
+      for (auto& sc : solo) {
+        Run& r = runs[sc.first];
+        if (r.h.done || sweep >= max_iters[r.k]) continue;
+        dispatch(r.j, [&](auto rr) { launch_bp_class<decltype(rr)>(r.j, r.j->cls[sc.second], sweep, sweep == 0); });
+      }
+    }
+    launched += c;
+    bool all = true;
+    for (auto& r : runs) {
+      if (r.h.done) continue;
+      HIPX(hipMemcpyAsync(r.j->st_host, r.j->status.p, sizeof(BpStatus), hipMemcpyDeviceToHost, b->stream));
+    }
+    HIPX(hipStreamSynchronize(b->stream));
+    for (auto& r : runs) {
+      if (r.h.done) continue;
+      apply_slot_rules(r.j->st_host, r.h, r.ruled, launched, max_iters[r.k], bp_eps[r.k]);
+      all = all && r.h.done;
+    }
+    if (all) break;
+    chunk = 4;
+  }
+  for (auto& r : runs) {
+    r.j->mcur = r.h.cur;
+    r.j->pred_sweeps = std::max(1, r.h.iters);
+    sweeps[r.k] = r.h.iters;
+    converged[r.k] = r.h.converged;
+    last_dist[r.k] = r.h.last_dist;
+    if (!r.h.have_best) status[r.k] = BQA_ERR_NUMERIC;  // bqa_run_bp's error for that job
+  }
+}
+
+}  // namespace
+
 // =================================================================== C ABI
 
 extern "C" {
@@ -1137,13 +1542,27 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
     j->E = edges_number / 2;
     j->chimax = max_bond_dim;
     HIPX(hipGetDevice(&j->device));
-    HIPX(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      HIPX(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
+    }
     {
       const char* g = std::getenv("BQA_GENERIC_KERNELS");
       if (g && g[0] == '1') j->generic_mask = ~0u;
       for (const char* q = g; q && *q; ++q)
         j->generic_mask |= *q == 'b' ? kGenBP : *q == 'd' ? kGenDensity : *q == 'e' ? kGenExt
                          : *q == 'c' ? kGenCanon : *q == 'l' ? kGenLocal : 0u;
+      const char* a = std::getenv("BQA_AUTOTUNE");
+      j->autotune = !(a && a[0] == '0');
+      const char* gr = std::getenv("BQA_GRAPHS");
+      j->use_graphs = !(gr && gr[0] == '0');
+      const char* v = std::getenv("BQA_BP_VARIANT");
+      if (v && *v) {
+        char* end = nullptr;
+        const long k = std::strtol(v, &end, 10);
+        require(end && *end == '\0' && k >= 0 && k < 16, "BQA_BP_VARIANT must be an integer in [0, 16)");
+        j->bp_variant_force = (int)k;
+      }
     }
     long total_nodes = 0, total_bonds = 0;
     for (int c = 0; c < classes; ++c) {
@@ -1191,14 +1610,14 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
         require(lmbd_pos[off_b + x] == input_pos[off_b + x] % (edges_number / 2),
                 "lambda position must equal the message position mod E");
       }
-      HIPX(hipMemcpy(cl.ids, node_ids + off_n, sizeof(int32_t) * cl.n, hipMemcpyHostToDevice));
+      HIPX(hipMemcpyAsync(cl.ids, node_ids + off_n, sizeof(int32_t) * cl.n, hipMemcpyHostToDevice, j->stream));
       if (nb) {
-        HIPX(hipMemcpy(cl.in, input_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-        HIPX(hipMemcpy(cl.out, output_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-        HIPX(hipMemcpy(cl.lp, lmbd_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-        HIPX(hipMemcpy(cl.J, edge_ampls + off_b, sizeof(double) * nb, hipMemcpyHostToDevice));
+        HIPX(hipMemcpyAsync(cl.in, input_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice, j->stream));
+        HIPX(hipMemcpyAsync(cl.out, output_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice, j->stream));
+        HIPX(hipMemcpyAsync(cl.lp, lmbd_pos + off_b, sizeof(int32_t) * nb, hipMemcpyHostToDevice, j->stream));
+        HIPX(hipMemcpyAsync(cl.J, edge_ampls + off_b, sizeof(double) * nb, hipMemcpyHostToDevice, j->stream));
       }
-      HIPX(hipMemcpy(cl.h, node_ampls + off_n, sizeof(double) * cl.n, hipMemcpyHostToDevice));
+      HIPX(hipMemcpyAsync(cl.h, node_ampls + off_n, sizeof(double) * cl.n, hipMemcpyHostToDevice, j->stream));
       off_n += cl.n;
       off_b += nb;
     }
@@ -1214,16 +1633,20 @@ int bqa_job_create(bqa_job** out, int32_t nodes_number, int32_t edges_number, in
         require(fwd_lhs[x] >= 0 && fwd_lhs[x] < nodes_number && fwd_rhs[x] >= 0 && fwd_rhs[x] < nodes_number,
                 "edge endpoint out of range");
       }
-      HIPX(hipMemcpy(lhs, fwd_lhs, sizeof(int32_t) * E, hipMemcpyHostToDevice));
-      HIPX(hipMemcpy(rhs, fwd_rhs, sizeof(int32_t) * E, hipMemcpyHostToDevice));
-      HIPX(hipMemcpy(jv, fwd_j, sizeof(double) * E, hipMemcpyHostToDevice));
-      HIPX(hipMemcpy(hv, h_by_id.data(), sizeof(double) * nodes_number, hipMemcpyHostToDevice));
+      HIPX(hipMemcpyAsync(lhs, fwd_lhs, sizeof(int32_t) * E, hipMemcpyHostToDevice, j->stream));
+      HIPX(hipMemcpyAsync(rhs, fwd_rhs, sizeof(int32_t) * E, hipMemcpyHostToDevice, j->stream));
+      HIPX(hipMemcpyAsync(jv, fwd_j, sizeof(double) * E, hipMemcpyHostToDevice, j->stream));
+      HIPX(hipMemcpyAsync(hv, h_by_id.data(), sizeof(double) * nodes_number, hipMemcpyHostToDevice, j->stream));
     }
     j->status.ensure(sizeof(BpStatus), j->dev_bytes);
     j->lmax.ensure(sizeof(double) * 128, j->dev_bytes);
     j->epart.ensure(sizeof(double) * 1024 + 64, j->dev_bytes);
-    HIPX(hipHostMalloc(reinterpret_cast<void**>(&j->st_host), sizeof(BpStatus), hipHostMallocDefault));
-    HIPX(hipHostMalloc(reinterpret_cast<void**>(&j->lmax_host), sizeof(double) * 128, hipHostMallocDefault));
+    {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      HIPX(hipHostMalloc(reinterpret_cast<void**>(&j->st_host), sizeof(BpStatus), hipHostMallocDefault));
+      HIPX(hipHostMalloc(reinterpret_cast<void**>(&j->lmax_host), sizeof(double) * 128, hipHostMallocDefault));
+    }
+    HIPX(hipStreamSynchronize(j->stream));  // the uploads read the caller's arrays and h_by_id
     *out = j;
   });
   if (rc != BQA_OK && j) bqa_job_destroy(j);
@@ -1235,6 +1658,7 @@ int bqa_job_destroy(bqa_job* j) {
   return guarded([&] {
     (void)hipSetDevice(j->device);
     if (j->stream) (void)hipStreamSynchronize(j->stream);
+    for (auto& g : j->graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto& c : j->cls) {
       c.layout.release(j->dev_bytes);
       c.T[0].release(j->dev_bytes);
@@ -1249,9 +1673,12 @@ int bqa_job_destroy(bqa_job* j) {
       (void)hipEventDestroy(pr.second);
     }
     for (auto e : j->ev_pool) (void)hipEventDestroy(e);
-    if (j->st_host) (void)hipHostFree(j->st_host);
-    if (j->lmax_host) (void)hipHostFree(j->lmax_host);
-    if (j->stream) (void)hipStreamDestroy(j->stream);
+    {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      if (j->st_host) (void)hipHostFree(j->st_host);
+      if (j->lmax_host) (void)hipHostFree(j->lmax_host);
+      if (j->stream) (void)hipStreamDestroy(j->stream);
+    }
     delete j;
   });
 }
@@ -1511,6 +1938,140 @@ int bqa_timing_read(bqa_job* j, int64_t* launches, double* total_ms, double* byt
   });
 }
 
+int bqa_job_graph_stats(const bqa_job* j, int64_t* captures, int64_t* launches, int64_t* failures) {
+  return guarded([&] {
+    require(j && captures && launches && failures, "null argument");
+    *captures = j->graph_captures;
+    *launches = j->graph_launches;
+    *failures = j->graph_failures;
+  });
+}
+
+int bqa_batch_create(bqa_batch** out, int32_t max_jobs) {
+  bqa_batch* b = nullptr;
+  int rc = guarded([&] {
+    require(out != nullptr, "null batch pointer");
+    require(max_jobs >= 1 && max_jobs <= 4096, "max_jobs must be in [1, 4096]");
+    b = new bqa_batch();
+    b->max_jobs = max_jobs;
+    HIPX(hipGetDevice(&b->device));
+    std::lock_guard<std::mutex> g(g_capture_mu);
+    HIPX(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b;
+  });
+  if (rc != BQA_OK && b) bqa_batch_destroy(b);
+  return rc;
+}
+
+int bqa_batch_destroy(bqa_batch* b) {
+  if (!b) return BQA_OK;
+  return guarded([&] {
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    b->tab.release(b->dev_bytes);
+    {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      if (b->tab_host) (void)hipHostFree(b->tab_host);
+      if (b->stream) (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+  });
+}
+
+int bqa_batch_run_layer(bqa_batch* b, int32_t count, bqa_job* const* jobs, const double* xtime, const double* ztime,
+                        const double* pinv_eps, const int32_t* max_iters, const double* bp_eps, const double* damping,
+                        int32_t* new_bond_dim, double* trunc_err, int32_t* sweeps, int32_t* converged,
+                        double* last_dist, int32_t* job_status) {
+  return guarded([&] {
+    require(b != nullptr, "null batch");
+    require(count >= 0 && count <= b->max_jobs, "count must be in [0, max_jobs]");
+    if (count == 0) return;
+    require(jobs && xtime && ztime && pinv_eps && max_iters && bp_eps && damping && new_bond_dim && trunc_err &&
+                sweeps && converged && last_dist && job_status,
+            "null argument");
+    for (int k = 0; k < count; ++k) job_status[k] = BQA_OK;
+    for (int k = 0; k < count; ++k) {
+      require(jobs[k] != nullptr, "null job");
+      require(jobs[k]->device == b->device, "the jobs of a batch run on the batch's device");
+      require(damping[k] >= 0.0 && damping[k] <= 1.0, "damping must be in [0, 1]");
+      require(!jobs[k]->timing, "kernel timing is not available in a batch");
+      for (int q = 0; q < k; ++q) require(jobs[q] != jobs[k], "a job appears twice in the batch");
+    }
+    HIPX(hipSetDevice(b->device));
+    // every job's earlier work is complete before its phases move to the batch stream
+    for (int k = 0; k < count; ++k) HIPX(hipStreamSynchronize(jobs[k]->stream));
+    std::vector<hipStream_t> own(count);
+    for (int k = 0; k < count; ++k) {
+      own[k] = jobs[k]->stream;
+      jobs[k]->stream = b->stream;
+    }
+    try {
+      // extended messages, canonicalisers and lambda maxima of every job; one synchronisation
+      for (int k = 0; k < count; ++k)
+        dispatch(jobs[k], [&](auto r) { ext_and_canon<decltype(r)>(jobs[k], ztime[k], pinv_eps[k], false); });
+      HIPX(hipStreamSynchronize(b->stream));
+      // the global truncation rule of each job, then its fused absorb + gates + gauge step
+      for (int k = 0; k < count; ++k)
+        dispatch(jobs[k], [&](auto r) {
+          bqa_job* j = jobs[k];
+          const int n2 = 2 * j->chi;
+          const int dim = truncation_dim(j, n2, pinv_eps[k], &trunc_err[k]);
+          j->lstride = n2;
+          local_update<decltype(r)>(j, kAbsorb | kGates | kGauge, dim, ztime[k], ztime[k], xtime[k]);
+          new_bond_dim[k] = dim;
+        });
+      batch_bp(b, count, jobs, max_iters, bp_eps, damping, sweeps, converged, last_dist, job_status);
+    } catch (...) {
+      (void)hipStreamSynchronize(b->stream);
+      for (int k = 0; k < count; ++k) jobs[k]->stream = own[k];
+      throw;
+    }
+    for (int k = 0; k < count; ++k) jobs[k]->stream = own[k];
+  });
+}
+
+int bqa_tuning_report(char* buf, int64_t capacity, int64_t* needed) {
+  return guarded([&] {
+    require(needed != nullptr && capacity >= 0 && (capacity == 0 || buf != nullptr), "null argument");
+    std::string out = "[";
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      bool firstrow = true;
+      for (const auto& kv : g_tune) {
+        const TuneKey& k = kv.first;
+        const TuneEntry& e = kv.second;
+        if (e.chosen < 0) continue;
+        char head[256];
+        std::snprintf(head, sizeof head,
+                      "%s{\"device\": %d, \"precision\": \"%s\", \"degree\": %d, \"chi\": %d, "
+                      "\"n_log2\": %d, \"chosen\": %d, \"variants\": [",
+                      firstrow ? "" : ", ", k.dev, k.prec == BQA_PREC_SINGLE ? "single" : "double", k.d, k.chi,
+                      k.nlog, e.chosen);
+        out += head;
+        for (size_t v = 0; v < e.us.size(); ++v) {
+          const char* name = k.prec == BQA_PREC_SINGLE ? bp_variant_name<float>(k.d, k.chi, (int)v)
+                                                       : bp_variant_name<double>(k.d, k.chi, (int)v);
+          char item[256];
+          std::snprintf(item, sizeof item, "%s{\"name\": \"%s\", \"us\": %.3f}", v ? ", " : "", name, e.us[v]);
+          out += item;
+        }
+        out += "]}";
+        firstrow = false;
+      }
+    }
+    out += "]";
+    *needed = (int64_t)out.size() + 1;
+    if (capacity >= *needed) std::memcpy(buf, out.c_str(), out.size() + 1);
+  });
+}
+
+int bqa_tuning_reset(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune.clear();
+  });
+}
+
 int bqa_sv_create(bqa_sv** out, int32_t qubits, const double* fields, int32_t edges_number, const int32_t* lhs,
                   const int32_t* rhs, const double* couplings) {
   bqa_sv* s = nullptr;
@@ -1529,7 +2090,10 @@ int bqa_sv_create(bqa_sv** out, int32_t qubits, const double* fields, int32_t ed
     s->n = qubits;
     s->E = edges_number;
     HIPX(hipGetDevice(&s->device));
-    HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    }
     const size_t dim = (size_t)1 << qubits;
     s->psi.ensure(dim * sizeof(cplx<double>), s->dev_bytes);
     s->ez.ensure(dim * sizeof(double), s->dev_bytes);
@@ -1541,11 +2105,11 @@ int bqa_sv_create(bqa_sv** out, int32_t qubits, const double* fields, int32_t ed
     double* jv = h + qubits;
     int32_t* a = reinterpret_cast<int32_t*>(jv + edges_number);
     int32_t* b = a + edges_number;
-    HIPX(hipMemcpy(h, fields, sizeof(double) * qubits, hipMemcpyHostToDevice));
+    HIPX(hipMemcpyAsync(h, fields, sizeof(double) * qubits, hipMemcpyHostToDevice, s->stream));
     if (edges_number) {
-      HIPX(hipMemcpy(jv, couplings, sizeof(double) * edges_number, hipMemcpyHostToDevice));
-      HIPX(hipMemcpy(a, lhs, sizeof(int32_t) * edges_number, hipMemcpyHostToDevice));
-      HIPX(hipMemcpy(b, rhs, sizeof(int32_t) * edges_number, hipMemcpyHostToDevice));
+      HIPX(hipMemcpyAsync(jv, couplings, sizeof(double) * edges_number, hipMemcpyHostToDevice, s->stream));
+      HIPX(hipMemcpyAsync(a, lhs, sizeof(int32_t) * edges_number, hipMemcpyHostToDevice, s->stream));
+      HIPX(hipMemcpyAsync(b, rhs, sizeof(int32_t) * edges_number, hipMemcpyHostToDevice, s->stream));
     }
     hipLaunchKernelGGL(k_sv_energy, dim3(grid_for((long)std::min<size_t>(dim, 1L << 30), kSvThreads, 16384)),
                        dim3(kSvThreads), 0, s->stream, reinterpret_cast<double*>(s->ez.p), qubits, h, edges_number, a,
@@ -1567,7 +2131,10 @@ int bqa_sv_destroy(bqa_sv* s) {
     s->ez.release(s->dev_bytes);
     s->graph.release(s->dev_bytes);
     s->part.release(s->dev_bytes);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream) {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      (void)hipStreamDestroy(s->stream);
+    }
     delete s;
   });
 }
@@ -1636,26 +2203,43 @@ int bqa_copy_calibrate(uint64_t bytes, int32_t repeats, double* ms_per_copy) {
   return guarded([&] {
     require(ms_per_copy && bytes >= 16 && repeats > 0, "bad argument");
     void *a = nullptr, *b = nullptr;
-    HIPX(hipMalloc(&a, bytes));
-    HIPX(hipMalloc(&b, bytes));
-    const long n = (long)(bytes / 16);
-    hipEvent_t e0, e1;
-    HIPX(hipEventCreate(&e0));
-    HIPX(hipEventCreate(&e1));
-    hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(n, 256, 16384)), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n);
-    HIPX(hipEventRecord(e0, 0));
-    for (int r = 0; r < repeats; ++r)
-      hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(n, 256, 16384)), dim3(256), 0, 0, (const uint4*)a, (uint4*)b,
-                         n);
-    HIPX(hipEventRecord(e1, 0));
-    HIPX(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPX(hipEventElapsedTime(&ms, e0, e1));
-    *ms_per_copy = ms / repeats;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (st) (void)hipStreamSynchronize(st);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (st) (void)hipStreamDestroy(st);
+    };
+    try {
+      {
+        std::lock_guard<std::mutex> g(g_capture_mu);
+        HIPX(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPX(hipMalloc(&a, bytes));
+        HIPX(hipMalloc(&b, bytes));
+      }
+      const long n = (long)(bytes / 16);
+      HIPX(hipEventCreate(&e0));
+      HIPX(hipEventCreate(&e1));
+      hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, (const uint4*)a, (uint4*)b, n);
+      HIPX(hipEventRecord(e0, st));
+      for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, (const uint4*)a,
+                           (uint4*)b, n);
+      LAUNCH_CHECK();
+      HIPX(hipEventRecord(e1, st));
+      HIPX(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPX(hipEventElapsedTime(&ms, e0, e1));
+      *ms_per_copy = ms / repeats;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
   });
 }
 

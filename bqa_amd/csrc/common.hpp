@@ -152,6 +152,7 @@ struct BpStatus {
   int damp_now;                 // the damping kernel must run after this sweep
   int max_iters;
   int marker;                   // slots: a launch found the run finished before its sweep
+  int base;                     // slots: added to every launch's sweep index (graph replays advance it)
   double bp_eps;
   double last_dist;
   double best_dist;
@@ -159,11 +160,14 @@ struct BpStatus {
 };
 
 // Buffer index a BP launch of sweep `sweep` reads, or -1 when the run is over.
-// With slots, a sweep runs unless an earlier one converged (the previous
-// sweep's distance below bp_eps, or a launch before it said so), and the
-// buffers alternate from the run's first one.
+// With slots (sweep >= 0), the launch's sweep is st->base + sweep; it runs
+// unless it is beyond the run's max_iters or an earlier sweep converged (the
+// previous sweep's distance below bp_eps, or a launch before it said so), and
+// the buffers alternate from the run's first one.
 __device__ inline int bp_begin(BpStatus* st, int sweep) {
   if (sweep < 0) return st->done ? -1 : st->cur;
+  sweep += st->base;
+  if (sweep >= st->max_iters) return -1;
   if (sweep > 0) {
     // blocks of one launch may set the marker while others read it; either
     // outcome gives the same decision, since both follow from the same slot
@@ -180,6 +184,7 @@ __device__ inline int bp_begin(BpStatus* st, int sweep) {
 
 // This block's distance maxima of sweep `sweep` (one thread per block).
 __device__ inline void bp_end(BpStatus* st, int sweep, double dnum, double dden) {
+  if (sweep >= 0) sweep += st->base;
   atomic_max_nonneg(sweep < 0 ? &st->num_bits : &st->slot[sweep][0], dnum);
   atomic_max_nonneg(sweep < 0 ? &st->den_bits : &st->slot[sweep][1], dden);
 }

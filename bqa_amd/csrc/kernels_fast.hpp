@@ -4,6 +4,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "launch_grid.hpp"
 #include "kernels_node.hpp"
 #include "kernels_mfma.hpp"
 #include "kernels_mfma16.hpp"
@@ -11,60 +12,29 @@
 
 namespace bqa {
 
-// Blocks of kernel K (THREADS per block) resident on the device at once: CUs x
-// resident blocks per CU, computed once per kernel.  The node kernels loop over
-// their nodes with a grid stride, so one resident round of blocks keeps every
-// CU busy to the end, where a larger grid leaves a partly filled last round.
-// The waves per CU are rounded down to a multiple of the CU's 4 SIMDs: with a
-// static split of the nodes, a SIMD holding one wave more than the others sets
-// the kernel's time (chi = 8 BP sweep at 100k nodes: 10 one-wave blocks per CU
-// 1.14 ms, 9 blocks 0.87 ms, 8 blocks 0.77 ms).
-template <auto K, int THREADS>
-inline int resident_blocks() {
-  static const int blocks = [] {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      return 4096;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, K, THREADS, 0) != hipSuccess || per <= 0) return 4096;
-    const int wpb = (THREADS + 63) / 64;
-    const int waves = per * wpb;
-    if (waves >= 4 && wpb <= 4 && 4 % wpb == 0) per = (waves - waves % 4) / wpb;
-    return cus * per;
-  }();
-  return blocks;
-}
-
-template <auto K, int THREADS>
-inline int node_grid(int n) {
-  const int r = resident_blocks<K, THREADS>();
-  return n < r ? n : r;
-}
-
-// Kernels with more than 64 KB of dynamic LDS: the size is allowed once per kernel, and the
-// resident-block count takes it into account.
-template <auto K, int THREADS, int LDS>
-inline int node_grid_lds(int n) {
-  static const int blocks = [] {
-    int dev = 0, cus = 0, per = 0;
-    if (LDS > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void*>(K),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS);
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, K, THREADS, LDS) != hipSuccess || per <= 0) per = 1;
-    return cus * per;
-  }();
-  return n < blocks ? n : blocks;
-}
-
 // Compile-time-shaped one-wave kernels (kernels_small.hpp) for every (d, chi)
 // they serve; the chi = 8 specialisations below (FP32 and FP64) use MFMA instead.
+//
+// The BP sweep has launch variants per (precision, degree, chi): kernels that split a node over
+// a different number of waves, and the grid policies of launch_grid.hpp.  bp_variant_count and
+// bp_variant_name describe them; fast_bp_launch(..., v) launches variant v (0 is the default),
+// and the autotuner in bqa_api.hip times them on the job's first sweep at each shape.
 template <typename R>
 bool fast_bp_launch(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                    hipStream_t s) {
-  return launch_small<kModeBP, R, R>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
+                    hipStream_t s, int v = 0);
+
+template <typename R>
+int bp_variant_count(int d, int chi);
+
+// Shapes whose BP sweep has a specialised kernel (fast_bp_launch returns true for them).
+template <typename R>
+inline bool bp_fast_shape(int d, int chi) {
+  if ((chi == 8 || (sizeof(R) == sizeof(float) && chi == 16)) && (d == 2 || d == 3)) return true;
+  return small_shape(d, chi);
 }
+
+template <typename R>
+const char* bp_variant_name(int d, int chi, int v);
 
 // The fused absorb/gates/gauge step at a fixed bond dimension: one-wave
 // kernels (kernels_small.hpp); the chi = 8, degree-3 case uses MFMA (both precisions).
@@ -113,59 +83,176 @@ bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msg
   return launch_small<kModeExt, R, double>(cv, chi, T, m, m, nullptr, ext, nullptr, nullptr, s);
 }
 
-// FP32, chi = 8, degree 2 or 3: MFMA kernels (kernels_mfma.hpp), one wave per node (two and
-// four waves per node for degree 3 measured no faster).
-template <int MODE>
-inline void launch_mfma8(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
-                         double* rho, double* bloch, hipStream_t s) {
-  if (cv.d == 2)
-    hipLaunchKernelGGL((k_mfma8<2, MODE, 1>), dim3(node_grid<k_mfma8<2, MODE, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
-                       mA, mB, st, rho, bloch);
-  else
-    hipLaunchKernelGGL((k_mfma8<3, MODE, 1>), dim3(node_grid<k_mfma8<3, MODE, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
-                       mA, mB, st, rho, bloch);
-}
-
 // FP32, chi = 16, degree 2 or 3: k_mfma16_bp, eight waves per degree-3 node (128 KB of LDS,
 // one node per CU), one wave per degree-2 node.
 template <int D, int W>
 inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
-                             hipStream_t s) {
+                             hipStream_t s, int policy) {
   constexpr int LDS = Mf16Lds<D, W>::BYTES;
-  const int grid = node_grid_lds<k_mfma16_bp<D, W>, 64 * W, LDS>(cv.n);
+  const int grid = node_grid_lds<k_mfma16_bp<D, W>, 64 * W, LDS>(cv.n, policy);
   hipLaunchKernelGGL((k_mfma16_bp<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, mA, mB, st);
+}
+
+// FP32 BP sweep variants.
+//   chi = 16, d = 3: k_mfma16_bp<3, W>, W = 8 or 4 waves per node (one node per CU, 128 KB LDS);
+//   chi = 16, d = 2: k_mfma16_bp<2, 1>, one or two resident rounds;
+//   chi = 8, d = 2, 3: k_mfma8 (one wave per node) under the three grid policies;
+//   other served shapes: k_small under the three grid policies.
+template <>
+inline int bp_variant_count<float>(int d, int chi) {
+  if (chi == 16 && (d == 2 || d == 3)) return 2;
+  if (chi == 8 && (d == 2 || d == 3)) return kGridPolicies;
+  return small_shape(d, chi) ? kGridPolicies : 1;
+}
+
+template <>
+inline const char* bp_variant_name<float>(int d, int chi, int v) {
+  if (chi == 16 && d == 3) return v == 0 ? "k_mfma16_bp<3,8>" : "k_mfma16_bp<3,4>";
+  if (chi == 16 && d == 2) return v == 0 ? "k_mfma16_bp<2,1>, resident" : "k_mfma16_bp<2,1>, 2x resident";
+  static const char* m8[kGridPolicies] = {"k_mfma8, resident SIMD-rounded", "k_mfma8, resident",
+                                          "k_mfma8, 2x resident"};
+  static const char* sm[kGridPolicies] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident"};
+  if (chi == 8 && (d == 2 || d == 3)) return m8[v % kGridPolicies];
+  return small_shape(d, chi) ? sm[v % kGridPolicies] : "generic";
+}
+
+// FP32, chi = 8, degree 2 or 3: k_mfma8, one wave per node (two and four waves per node for
+// degree 3 measured no faster).
+template <int MODE, int D>
+inline void launch_mfma8_d(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
+                           double* rho, double* bloch, hipStream_t s, int policy) {
+  hipLaunchKernelGGL((k_mfma8<D, MODE, 1>), dim3(policy_grid<k_mfma8<D, MODE, 1>, 64>(cv.n, policy)), dim3(64), 0, s,
+                     cv, T, mA, mB, st, rho, bloch);
 }
 
 template <>
 inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB,
-                                  BpStatus* st, hipStream_t s) {
+                                  BpStatus* st, hipStream_t s, int v) {
   if (chi == 16 && cv.n > 0 && (cv.d == 2 || cv.d == 3)) {
-    if (cv.d == 3)
-      launch_mfma16_bp<3, 8>(cv, T, mA, mB, st, s);
-    else
-      launch_mfma16_bp<2, 1>(cv, T, mA, mB, st, s);
+    if (cv.d == 3) {
+      if (v == 1)
+        launch_mfma16_bp<3, 4>(cv, T, mA, mB, st, s, 1);
+      else
+        launch_mfma16_bp<3, 8>(cv, T, mA, mB, st, s, 1);
+    } else {
+      launch_mfma16_bp<2, 1>(cv, T, mA, mB, st, s, v == 1 ? 2 : 1);
+    }
     return true;
   }
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
-    return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
-  launch_mfma8<kModeBP>(cv, T, mA, mB, st, nullptr, nullptr, s);
+    return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s, v);
+  if (cv.d == 2)
+    launch_mfma8_d<kModeBP, 2>(cv, T, mA, mB, st, nullptr, nullptr, s, v);
+  else
+    launch_mfma8_d<kModeBP, 3>(cv, T, mA, mB, st, nullptr, nullptr, s, v);
   return true;
 }
 
-// FP64, chi = 8: FP64 MFMA; degree 3 in six contractions per node over four
-// waves (k_mfma8_bp64_3p), degree 2 one wave per node (k_mfma8_bp64).
+// FP64 BP sweep variants.
+//   chi = 8, d = 3: k_mfma8_bp64_3p<4> (six contractions per node over four waves) under the three
+//     grid policies, and k_mfma8_bp64_3p<2> (two waves per node);
+//   chi = 8, d = 2: k_mfma8_bp64<2, 1> (one wave per node) under the three grid policies;
+//   other served shapes: k_small under the three grid policies.
+template <>
+inline int bp_variant_count<double>(int d, int chi) {
+  if (chi == 8 && d == 3) return kGridPolicies + 1;
+  if (chi == 8 && d == 2) return kGridPolicies;
+  return small_shape(d, chi) ? kGridPolicies : 1;
+}
+
+template <>
+inline const char* bp_variant_name<double>(int d, int chi, int v) {
+  static const char* b3[kGridPolicies + 1] = {"k_mfma8_bp64_3p<4>, resident SIMD-rounded",
+                                              "k_mfma8_bp64_3p<4>, resident", "k_mfma8_bp64_3p<4>, 2x resident",
+                                              "k_mfma8_bp64_3p<2>, resident SIMD-rounded"};
+  static const char* b2[kGridPolicies] = {"k_mfma8_bp64<2,1>, resident SIMD-rounded", "k_mfma8_bp64<2,1>, resident",
+                                          "k_mfma8_bp64<2,1>, 2x resident"};
+  static const char* sm[kGridPolicies] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident"};
+  if (chi == 8 && d == 3) return b3[v % (kGridPolicies + 1)];
+  if (chi == 8 && d == 2) return b2[v % kGridPolicies];
+  return small_shape(d, chi) ? sm[v % kGridPolicies] : "generic";
+}
+
 template <>
 inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
-                                   BpStatus* st, hipStream_t s) {
+                                   BpStatus* st, hipStream_t s, int v) {
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
-    return launch_small<kModeBP, double, double>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s);
-  if (cv.d == 3)
-    hipLaunchKernelGGL((k_mfma8_bp64_3p<4>), dim3(node_grid<k_mfma8_bp64_3p<4>, 256>(cv.n)), dim3(256), 0, s, cv, T,
-                       mA, mB, st);
-  else
-    hipLaunchKernelGGL((k_mfma8_bp64<2, 1>), dim3(node_grid<k_mfma8_bp64<2, 1>, 64>(cv.n)), dim3(64), 0, s, cv, T,
-                       mA, mB, st);
+    return launch_small<kModeBP, double, double>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s, v);
+  if (cv.d == 3) {
+    if (v == kGridPolicies)
+      hipLaunchKernelGGL((k_mfma8_bp64_3p<2>), dim3(node_grid<k_mfma8_bp64_3p<2>, 128>(cv.n)), dim3(128), 0, s, cv,
+                         T, mA, mB, st);
+    else
+      hipLaunchKernelGGL((k_mfma8_bp64_3p<4>), dim3(policy_grid<k_mfma8_bp64_3p<4>, 256>(cv.n, v)), dim3(256), 0, s,
+                         cv, T, mA, mB, st);
+  } else {
+    hipLaunchKernelGGL((k_mfma8_bp64<2, 1>), dim3(policy_grid<k_mfma8_bp64<2, 1>, 64>(cv.n, v)), dim3(64), 0, s, cv,
+                       T, mA, mB, st);
+  }
   return true;
+}
+
+// Multi-job BP launches (the serving batch): which kernel serves a class of a job of
+// precision R in a launch shared with other jobs -- 1 k_small_multi, 2 k_mfma8_multi, or 0
+// when the class keeps its own launches.  Only shapes whose single-job launch runs the same
+// node code are shared, so a job's results are the same in a batch as alone.
+template <typename R>
+inline int multi_family(int d, int chi) {
+  if (sizeof(R) == sizeof(float) && chi == 8 && (d == 2 || d == 3)) return 2;
+  if (sizeof(R) == sizeof(double) && chi == 8 && (d == 2 || d == 3)) return 0;  // FP64 MFMA kernels
+  return small_shape(d, chi) ? 1 : 0;
+}
+
+template <typename R, int D, int CHI>
+inline void launch_small_multi_dc(const BpEntry* tab, int entries, int total, int sweep, hipStream_t s) {
+  hipLaunchKernelGGL((k_small_multi<D, CHI, R>), dim3(policy_grid<k_small_multi<D, CHI, R>, 64>(total, 0)), dim3(64),
+                     0, s, tab, entries, sweep);
+}
+
+template <typename R, int D>
+inline bool launch_small_multi_d(int chi, const BpEntry* tab, int entries, int total, int sweep, hipStream_t s) {
+  switch (chi) {
+    case 2: launch_small_multi_dc<R, D, 2>(tab, entries, total, sweep, s); return true;
+    case 3: launch_small_multi_dc<R, D, 3>(tab, entries, total, sweep, s); return true;
+    case 4: launch_small_multi_dc<R, D, 4>(tab, entries, total, sweep, s); return true;
+    default: break;
+  }
+  if constexpr (D <= 3) {
+    switch (chi) {
+      case 5: launch_small_multi_dc<R, D, 5>(tab, entries, total, sweep, s); return true;
+      case 6: launch_small_multi_dc<R, D, 6>(tab, entries, total, sweep, s); return true;
+      case 7: launch_small_multi_dc<R, D, 7>(tab, entries, total, sweep, s); return true;
+      case 8: launch_small_multi_dc<R, D, 8>(tab, entries, total, sweep, s); return true;
+      default: break;
+    }
+  }
+  return false;
+}
+
+// One shared launch of sweep `sweep` over `entries` classes (same family, degree and chi)
+// holding `total` nodes; false if the family has no such kernel.
+template <typename R>
+inline bool launch_bp_multi(int fam, int d, int chi, const BpEntry* tab, int entries, int total, int sweep,
+                            hipStream_t s) {
+  if (fam == 2) {
+    if constexpr (sizeof(R) == sizeof(float)) {
+      if (d == 2)
+        hipLaunchKernelGGL((k_mfma8_multi<2>), dim3(policy_grid<k_mfma8_multi<2>, 64>(total, 0)), dim3(64), 0, s, tab,
+                           entries, sweep);
+      else
+        hipLaunchKernelGGL((k_mfma8_multi<3>), dim3(policy_grid<k_mfma8_multi<3>, 64>(total, 0)), dim3(64), 0, s, tab,
+                           entries, sweep);
+      return true;
+    }
+    return false;
+  }
+  if (fam != 1) return false;
+  switch (d) {
+    case 2: return launch_small_multi_d<R, 2>(chi, tab, entries, total, sweep, s);
+    case 3: return launch_small_multi_d<R, 3>(chi, tab, entries, total, sweep, s);
+    case 4: return launch_small_multi_d<R, 4>(chi, tab, entries, total, sweep, s);
+    default: return false;
+  }
 }
 
 template <>
@@ -174,7 +261,10 @@ inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>*
   cplx<float>* m = const_cast<cplx<float>*>(msgs);
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeDensity, float, float>(cv, chi, T, m, m, nullptr, nullptr, rho, bloch, s);
-  launch_mfma8<kModeDensity>(cv, T, m, m, nullptr, rho, bloch, s);
+  if (cv.d == 2)
+    launch_mfma8_d<kModeDensity, 2>(cv, T, m, m, nullptr, rho, bloch, s, 0);
+  else
+    launch_mfma8_d<kModeDensity, 3>(cv, T, m, m, nullptr, rho, bloch, s, 0);
   return true;
 }
 

@@ -322,10 +322,11 @@ __device__ __forceinline__ void node_sync() {
     __syncthreads();
 }
 
+// One block's share of a class: nodes start, start + gridDim.x, ...
 template <int D, int MODE, int W>
-__global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
-                                               cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
-                                               double* rho_out, double* bloch_out) {
+__device__ __forceinline__ void mf8_run(const ClassView& cv, const cplx<float>* __restrict__ T, cplx<float>* msgA,
+                                        cplx<float>* msgB, BpStatus* st, double* rho_out, double* bloch_out,
+                                        int start) {
   static_assert(D == 2 || D == 3, "degree 2 or 3");
   static_assert(MODE == kModeBP || MODE == kModeDensity, "BP or density");
   __shared__ __align__(16) Mf8Smem<D, W> sm;
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
   double dnum = 0.0, dden = 0.0;
   constexpr int S = Mf8<D>::S;
   const Swz8Lanes<D> zl = swz8_lanes<D, W>();
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+  for (int i = start; i < n; i += gridDim.x) {
     load_tensor8<D, W>(T + (size_t)i * S, sm.T);
     float bm[D][4];
 #pragma unroll
@@ -471,6 +472,28 @@ __global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float
     if (l == 0) {
       bp_end(st, cv.sweep, dnum, dden);
     }
+  }
+}
+
+template <int D, int MODE, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma8(ClassView cv, const cplx<float>* __restrict__ T,
+                                               cplx<float>* msgA, cplx<float>* msgB, BpStatus* st,
+                                               double* rho_out, double* bloch_out) {
+  mf8_run<D, MODE, W>(cv, T, msgA, msgB, st, rho_out, bloch_out, blockIdx.x);
+}
+
+// One BP sweep of the chi = 8 classes of several FP32 jobs in one launch, nodes shared over
+// the blocks as in k_small_multi.
+template <int D>
+__global__ __launch_bounds__(64) void k_mfma8_multi(const BpEntry* __restrict__ tab, int entries, int sweep) {
+  for (int e = 0; e < entries; ++e) {
+    ClassView cv = tab[e].cv;
+    cv.sweep = sweep;
+    const long g = gridDim.x;
+    const int start = (int)((((long)blockIdx.x - tab[e].prefix) % g + g) % g);
+    mf8_run<D, kModeBP, 1>(cv, reinterpret_cast<const cplx<float>*>(tab[e].T),
+                           reinterpret_cast<cplx<float>*>(tab[e].mA), reinterpret_cast<cplx<float>*>(tab[e].mB),
+                           tab[e].st, nullptr, nullptr, start);
   }
 }
 

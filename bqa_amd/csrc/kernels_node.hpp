@@ -29,6 +29,18 @@ struct ClassView {
   int sweep;
 };
 
+// One degree class of one job in a launch that covers several jobs (the serving
+// batch): the class, its tensors and message buffers, its job's status block, and
+// the nodes of the launch's earlier entries.
+struct BpEntry {
+  ClassView cv;
+  const void* T;
+  void* mA;
+  void* mB;
+  BpStatus* st;
+  long prefix;
+};
+
 enum NodeMode { kModeBP = 0, kModeExt = 1, kModeDensity = 2 };
 
 // dst[.., a, ..] = sum_b m[a, b] src[.., b, ..] along the axis of `stride`.

@@ -17,6 +17,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "launch_grid.hpp"
 #include "kernels_node.hpp"
 
 namespace bqa {
@@ -199,10 +200,11 @@ __device__ __forceinline__ void sout(SmallSmem<D, CHI, C>& sm, const cplx<C>* Y,
   }
 }
 
+// One block's share of a class: nodes start, start + gridDim.x, ...
 template <int D, int CHI, int MODE, typename R, typename C>
-__global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
-                                              cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out,
-                                              double* rho_out, double* bloch_out) {
+__device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
+                                          cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out, double* rho_out,
+                                          double* bloch_out, int start) {
   static_assert(D >= 2 && D <= 4 && CHI >= 2 && CHI <= 8, "shape");
   using M = Sm<D, CHI>;
   __shared__ __align__(16) SmallSmem<D, CHI, C> sm;
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
     mout = cur ? msgA : msgB;
   }
   double dnum = 0.0, dden = 0.0;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+  for (int i = start; i < n; i += gridDim.x) {
     const cplx<R>* Tg = T + (size_t)i * M::S;
 #pragma unroll
     for (int q = 0; q < (M::S + 63) / 64; ++q) {
@@ -345,6 +347,30 @@ __global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __res
   }
 }
 
+template <int D, int CHI, int MODE, typename R, typename C>
+__global__ __launch_bounds__(64) void k_small(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
+                                              cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out,
+                                              double* rho_out, double* bloch_out) {
+  small_run<D, CHI, MODE, R, C>(cv, T, msgA, msgB, st, ext_out, rho_out, bloch_out, blockIdx.x);
+}
+
+// One BP sweep of the classes of several jobs (same degree and chi) in one launch: entry e's
+// node i is flat node prefix_e + i, taken by block (prefix_e + i) mod gridDim.x, so the
+// blocks share the nodes of all entries evenly.  Each entry has its own status block, so
+// every job keeps its own convergence and buffer bookkeeping.
+template <int D, int CHI, typename R>
+__global__ __launch_bounds__(64) void k_small_multi(const BpEntry* __restrict__ tab, int entries, int sweep) {
+  for (int e = 0; e < entries; ++e) {
+    ClassView cv = tab[e].cv;
+    cv.sweep = sweep;
+    const long g = gridDim.x;
+    const int start = (int)((((long)blockIdx.x - tab[e].prefix) % g + g) % g);
+    small_run<D, CHI, kModeBP, R, R>(cv, reinterpret_cast<const cplx<R>*>(tab[e].T),
+                                     reinterpret_cast<cplx<R>*>(tab[e].mA), reinterpret_cast<cplx<R>*>(tab[e].mB),
+                                     tab[e].st, nullptr, nullptr, nullptr, start);
+  }
+}
+
 // Shapes served: degree 2 and 3 with chi 2..8, degree 4 with chi 2..4.
 inline bool small_shape(int d, int chi) {
   return chi >= 2 && ((d == 2 || d == 3) ? chi <= 8 : (d == 4 && chi <= 4));
@@ -352,28 +378,35 @@ inline bool small_shape(int d, int chi) {
 
 inline int small_grid(int n) { return n < 8192 ? n : 8192; }
 
+// Grid of k_small under launch policy `policy`: 0 at most 8192 one-wave blocks,
+// otherwise the grid policies of launch_grid.hpp (1 one resident round, 2 two).
+template <auto K>
+inline int small_policy_grid(int n, int policy) {
+  return policy == 0 ? small_grid(n) : policy_grid<K, 64>(n, policy);
+}
+
 template <int D, int CHI, int MODE, typename R, typename C>
 inline void launch_small_dc(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                            cplx<double>* ext, double* rho, double* bloch, hipStream_t s) {
-  hipLaunchKernelGGL((k_small<D, CHI, MODE, R, C>), dim3(small_grid(cv.n)), dim3(64), 0, s, cv, T, mA, mB, st, ext,
-                     rho, bloch);
+                            cplx<double>* ext, double* rho, double* bloch, hipStream_t s, int policy) {
+  hipLaunchKernelGGL((k_small<D, CHI, MODE, R, C>), dim3(small_policy_grid<k_small<D, CHI, MODE, R, C>>(cv.n, policy)),
+                     dim3(64), 0, s, cv, T, mA, mB, st, ext, rho, bloch);
 }
 
 template <int D, int MODE, typename R, typename C>
 inline bool launch_small_d(int chi, ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                           cplx<double>* ext, double* rho, double* bloch, hipStream_t s) {
+                           cplx<double>* ext, double* rho, double* bloch, hipStream_t s, int policy) {
   switch (chi) {
-    case 2: launch_small_dc<D, 2, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
-    case 3: launch_small_dc<D, 3, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
-    case 4: launch_small_dc<D, 4, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+    case 2: launch_small_dc<D, 2, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
+    case 3: launch_small_dc<D, 3, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
+    case 4: launch_small_dc<D, 4, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
     default: break;
   }
   if constexpr (D <= 3) {
     switch (chi) {
-      case 5: launch_small_dc<D, 5, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
-      case 6: launch_small_dc<D, 6, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
-      case 7: launch_small_dc<D, 7, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
-      case 8: launch_small_dc<D, 8, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s); return true;
+      case 5: launch_small_dc<D, 5, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
+      case 6: launch_small_dc<D, 6, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
+      case 7: launch_small_dc<D, 7, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
+      case 8: launch_small_dc<D, 8, MODE, R, C>(cv, T, mA, mB, st, ext, rho, bloch, s, policy); return true;
       default: break;
     }
   }
@@ -384,12 +417,12 @@ inline bool launch_small_d(int chi, ClassView cv, const cplx<R>* T, cplx<R>* mA,
 // extended-message mode reads the ZZ-split gates from cv.zz.
 template <int MODE, typename R, typename C>
 inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
-                         cplx<double>* ext, double* rho, double* bloch, hipStream_t s) {
+                         cplx<double>* ext, double* rho, double* bloch, hipStream_t s, int policy = 0) {
   if (cv.n == 0 || !small_shape(cv.d, chi)) return false;
   switch (cv.d) {
-    case 2: return launch_small_d<2, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s);
-    case 3: return launch_small_d<3, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s);
-    case 4: return launch_small_d<4, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s);
+    case 2: return launch_small_d<2, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
+    case 3: return launch_small_d<3, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
+    case 4: return launch_small_d<4, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
     default: return false;
   }
 }

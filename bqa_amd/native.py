@@ -7,6 +7,7 @@ there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 from ctypes import POINTER, c_char_p, c_double, c_int, c_int8, c_int32, c_int64, c_uint64, c_void_p
 
@@ -58,6 +59,13 @@ SIGNATURES = {
     "bqa_timing_enable": (c_int, [c_void_p, c_int32]),
     "bqa_timing_read": (c_int, [c_void_p, POINTER(c_int64), _f64p, _f64p]),
     "bqa_copy_calibrate": (c_int, [c_uint64, c_int32, _f64p]),
+    "bqa_tuning_report": (c_int, [ctypes.c_char_p, c_int64, POINTER(c_int64)]),
+    "bqa_tuning_reset": (c_int, []),
+    "bqa_job_graph_stats": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64)]),
+    "bqa_batch_create": (c_int, [POINTER(c_void_p), c_int32]),
+    "bqa_batch_destroy": (c_int, [c_void_p]),
+    "bqa_batch_run_layer": (c_int, [c_void_p, c_int32, POINTER(c_void_p), _f64p, _f64p, _f64p, _i32p, _f64p, _f64p,
+                                    _i32p, _f64p, _i32p, _i32p, _f64p, _i32p]),
     "bqa_sv_create": (c_int, [POINTER(c_void_p), c_int32, _f64p, c_int32, _i32p, _i32p, _f64p]),
     "bqa_sv_destroy": (c_int, [c_void_p]),
     "bqa_sv_init": (c_int, [c_void_p]),
@@ -303,10 +311,59 @@ class Job:
     def timing_enable(self, on: bool) -> None:
         check(LIB.bqa_timing_enable(self.handle, 1 if on else 0))
 
+    def graph_stats(self) -> tuple[int, int, int]:
+        """(graphs captured, graph replays, failed captures) of this job's BP chunks."""
+        c, n, f = c_int64(0), c_int64(0), c_int64(0)
+        check(LIB.bqa_job_graph_stats(self.handle, ctypes.byref(c), ctypes.byref(n), ctypes.byref(f)))
+        return c.value, n.value, f.value
+
     def timing_read(self) -> tuple[int, float, float]:
         n, ms, b = c_int64(0), c_double(0.0), c_double(0.0)
         check(LIB.bqa_timing_read(self.handle, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
         return n.value, ms.value, b.value
+
+
+class Batch:
+    """A serving batch (``bqa_batch``): one Trotter step of several jobs per call, their phases on one
+    stream and their BP sweeps in shared launches per (precision, degree, chi) shape.  Each job's
+    results are those of ``Job.run_layer`` on it alone."""
+
+    def __init__(self, max_jobs: int):
+        h = c_void_p()
+        check(LIB.bqa_batch_create(ctypes.byref(h), int(max_jobs)))
+        self.handle = h
+        self.max_jobs = int(max_jobs)
+
+    def run_layer(self, jobs: list, xtime, ztime, pinv_eps, max_iters, bp_eps, damping) -> list:
+        """One Trotter step of every job; per-job arguments are sequences (or scalars for all).  Returns one
+        dict per job as ``Job.run_layer`` does, plus ``error`` (None, or the message of the job's BP failure:
+        no finite distance)."""
+        k = len(jobs)
+        if k == 0:
+            return []
+        vec = lambda v, dt: np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=dt), (k,)))  # noqa: E731
+        xt, zt, pe, be, dm = (vec(v, np.float64) for v in (xtime, ztime, pinv_eps, bp_eps, damping))
+        it = vec(max_iters, np.int32)
+        handles = (c_void_p * k)(*[j.handle for j in jobs])
+        dim, sw, cv, st = (np.zeros(k, np.int32) for _ in range(4))
+        err, dist = np.zeros(k), np.zeros(k)
+        f, i = (lambda a: a.ctypes.data_as(_f64p)), (lambda a: a.ctypes.data_as(_i32p))
+        check(LIB.bqa_batch_run_layer(self.handle, k, handles, f(xt), f(zt), f(pe), i(it), f(be), f(dm), i(dim),
+                                      f(err), i(sw), i(cv), f(dist), i(st)))
+        return [{"bond_dim": int(dim[q]), "trunc_err": float(err[q]), "sweeps": int(sw[q]),
+                 "converged": bool(cv[q]), "dist": float(dist[q]),
+                 "error": None if st[q] == BQA_OK else "BP produced no finite message distance"} for q in range(k)]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(LIB.bqa_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def energy_noalloc(bloch_dev: int, nodes_number: int, fields_dev: int, edges_number: int, lhs_dev: int,
@@ -315,6 +372,21 @@ def energy_noalloc(bloch_dev: int, nodes_number: int, fields_dev: int, edges_num
     check(LIB.bqa_energy_noalloc(c_void_p(bloch_dev), int(nodes_number), c_void_p(fields_dev), int(edges_number),
                                  c_void_p(lhs_dev), c_void_p(rhs_dev), c_void_p(j_dev), c_void_p(energy_dev),
                                  c_void_p(stream)))
+
+
+def tuning_report() -> list:
+    """The BP launch autotuner's table (``bqa_tuning_report``): one dict per timed shape with the
+    variants' names and microseconds per launch, and the index chosen."""
+    need = c_int64(0)
+    check(LIB.bqa_tuning_report(None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(max(1, need.value))
+    check(LIB.bqa_tuning_report(buf, len(buf), ctypes.byref(need)))
+    return json.loads(buf.value.decode())
+
+
+def tuning_reset() -> None:
+    """Forget every autotuner choice of this process (``bqa_tuning_reset``)."""
+    check(LIB.bqa_tuning_reset())
 
 
 def copy_calibrate(nbytes: int, repeats: int = 20) -> float:

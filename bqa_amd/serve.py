@@ -3,10 +3,18 @@
 Each request is a reference config (the keys ``run_qa`` takes, core.py:13-35).
 Jobs are admitted in arrival order once they have arrived, a slot is free
 (at most ``max_concurrent`` jobs resident) and their device footprint
-(``bqa_job_estimate_bytes``) fits the remaining HBM budget.  Every admitted
-job runs its instruction stream on its own host thread with its own
-``bqa_job`` handle and HIP stream, so the sweeps of concurrent jobs overlap on
-the GPU; a job's results are those it gives alone (the kernels are
+(``bqa_job_estimate_bytes``) fits the remaining HBM budget.  Two executors:
+
+* batched (the default): one host thread steps every resident job together
+  through a ``bqa_batch`` -- each Trotter step of all resident jobs is one call,
+  with the jobs' canonicalisers before one synchronisation and their BP sweeps
+  in launches shared per (precision, degree, chi) shape; a job joins the batch
+  at its next step once admitted and leaves it after its last instruction;
+* threaded: every job runs its instruction stream on its own host thread with
+  its own ``bqa_job`` handle and HIP stream, so the sweeps of concurrent jobs
+  overlap on the GPU.
+
+Either way a job's results are those it gives alone (the kernels are
 deterministic and jobs share no device state).
 
 ``serve`` returns each job's results with its arrival, start and finish
@@ -77,17 +85,40 @@ def _percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=float), q)) if len(v) else None
 
 
+class NativeBackend:
+    """Jobs and batches of ``libbqa_hip.so`` for the batched executor."""
+
+    @staticmethod
+    def make_job(ctx, precision):
+        from .native import Job
+        job = Job(ctx, precision)
+        try:
+            job.init_state()
+        except Exception:
+            job.close()
+            raise
+        return job
+
+    @staticmethod
+    def make_batch(max_jobs):
+        from .native import Batch
+        return Batch(max_jobs)
+
+
 def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGET, precision: str = "single",
-          runner=None, footprint=None) -> dict:
+          runner=None, footprint=None, batched: bool = True, backend=None) -> dict:
     """Run the requests through the admission loop; see the module docstring.
 
-    ``runner(ctx, precision) -> (results, steps, device_bytes)`` and
-    ``footprint(ctx, precision) -> bytes`` default to the GPU job and
-    ``bqa_job_estimate_bytes``; they are parameters so the scheduling can be
-    exercised without a device.
+    ``footprint(ctx, precision) -> bytes`` defaults to ``bqa_job_estimate_bytes``.  The
+    batched executor takes its jobs and batch from ``backend`` (``NativeBackend``); the
+    threaded one (``batched=False``, or any ``runner``) runs
+    ``runner(ctx, precision) -> (results, steps, device_bytes)``, by default ``run_job``.
+    Both are parameters so the scheduling can be exercised without a device.
     """
     if max_concurrent < 1:
         raise ValueError("max_concurrent must be at least 1")
+    if runner is not None:
+        batched = False
     if runner is None:
         runner = run_job
     if footprint is None:
@@ -101,6 +132,9 @@ def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGE
         if need > hbm_budget:
             raise MemoryError(f"request {req.tag!r} needs {need} bytes, more than the HBM budget {hbm_budget}")
         prepared.append((idx, req, ctx, need))
+    if batched:
+        outs = _serve_batched(prepared, max_concurrent, hbm_budget, precision, backend or NativeBackend)
+        return _summary(outs, max_concurrent)
     outcomes: dict[int, Outcome] = {}
     lock = threading.Condition()
     state = {"running": 0, "bytes": 0}
@@ -139,7 +173,99 @@ def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGE
                 lock.wait(timeout=timeout if timeout is not None else 1.0)
     for th in threads:
         th.join()
-    outs = [outcomes[i] for i in sorted(outcomes)]
+    return _summary([outcomes[i] for i in sorted(outcomes)], max_concurrent)
+
+
+def _serve_batched(prepared, max_concurrent, hbm_budget, precision, backend) -> list:
+    """The batched executor: admission as in the threaded one, then every resident job's next Trotter
+    step in one ``Batch.run_layer`` call per round; readouts between steps run on the job itself."""
+    batch = backend.make_batch(max_concurrent)
+    outcomes: dict[int, Outcome] = {}
+    queue = list(prepared)
+    resident = []
+    used = 0
+    t0 = time.perf_counter()
+
+    def finish(r, err):
+        nonlocal used
+        dbytes = 0
+        try:
+            dbytes = r["job"].device_bytes()
+        except Exception:
+            pass
+        try:
+            r["job"].close()
+        except Exception as e:  # reported with the job
+            err = err or f"{type(e).__name__}: {e}"
+        fin = time.perf_counter() - t0
+        req, ctx = r["req"], r["ctx"]
+        outcomes[r["idx"]] = Outcome(req.tag, r["results"] if err is None else [], req.arrival, r["start"], fin,
+                                     ctx.nodes_number, r["steps"], dbytes, err)
+        used -= r["need"]
+        resident.remove(r)
+
+    try:
+        while queue or resident:
+            now = time.perf_counter() - t0
+            while queue and queue[0][1].arrival <= now and len(resident) < max_concurrent and \
+                    used + queue[0][3] <= hbm_budget:
+                idx, req, ctx, need = queue.pop(0)
+                try:
+                    job = backend.make_job(ctx, precision)
+                except Exception as e:  # recorded per job; the loop keeps serving the others
+                    outcomes[idx] = Outcome(req.tag, [], req.arrival, now, time.perf_counter() - t0,
+                                            ctx.nodes_number, 0, 0, f"{type(e).__name__}: {e}")
+                    continue
+                resident.append({"idx": idx, "req": req, "ctx": ctx, "need": need, "start": now, "job": job,
+                                 "pc": 0, "steps": 0, "results": []})
+                used += need
+            if not resident:
+                if queue:
+                    time.sleep(max(0.0, min(1.0, queue[0][1].arrival - (time.perf_counter() - t0))))
+                continue
+            stepping = []
+            for r in list(resident):
+                ins = r["ctx"].instructions
+                try:
+                    while r["pc"] < len(ins) and not isinstance(ins[r["pc"]], dict):
+                        if ins[r["pc"]] != "get_bloch_vectors":
+                            raise NotImplementedError(f"instruction {ins[r['pc']]!r} is not served")
+                        r["results"].append(["bloch_vectors", r["job"].density()[1].tolist()])
+                        r["pc"] += 1
+                except Exception as e:
+                    finish(r, f"{type(e).__name__}: {e}")
+                    continue
+                if r["pc"] >= len(ins):
+                    finish(r, None)
+                else:
+                    stepping.append(r)
+            if not stepping:
+                continue
+            lays = [r["ctx"].instructions[r["pc"]] for r in stepping]
+            cx = [r["ctx"] for r in stepping]
+            try:
+                outs = batch.run_layer([r["job"] for r in stepping], [l["xtime"] for l in lays],
+                                       [l["ztime"] for l in lays], [c.pinv_eps for c in cx],
+                                       [c.max_bp_iters_number for c in cx], [c.bp_eps for c in cx],
+                                       [c.damping for c in cx])
+            except Exception as e:  # the whole step failed: every job in it is reported
+                for r in stepping:
+                    finish(r, f"{type(e).__name__}: {e}")
+                continue
+            for r, o in zip(stepping, outs):
+                if o.get("error"):
+                    finish(r, f"AssertionError: {o['error']}")
+                else:
+                    r["pc"] += 1
+                    r["steps"] += 1
+    finally:
+        for r in list(resident):
+            finish(r, "RuntimeError: the serving loop stopped")
+        batch.close()
+    return [outcomes[i] for i in sorted(outcomes)]
+
+
+def _summary(outs, max_concurrent) -> dict:
     ok = [o for o in outs if o.error is None]
     makespan = max((o.finish for o in outs), default=0.0)
     lat = [o.latency for o in ok]
@@ -158,10 +284,11 @@ def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGE
 
 
 def mixed_requests(count: int, seed: int = 5, n_min: int = 100, n_max: int = 5000, rate_per_s: float = 8.0,
-                   degrees=(3, 4), chis=(4, 8), total_time: float = 10.0) -> list[Request]:
+                   degrees=(3, 4), chis=(4, 8), total_time: float = 10.0, chi_cap=None) -> list[Request]:
     """Seeded mixed workload: Poisson arrivals at ``rate_per_s``; n log-uniform in [n_min, n_max] (n d even);
-    degree and bond dimension drawn from ``degrees`` and ``chis``; J ~ U[-1, 1], h = 0; Trotter steps
-    100..300 growing with log n (``instances.serving_steps``)."""
+    degree and bond dimension drawn from ``degrees`` and ``chis`` (the bond dimension then capped at
+    ``chi_cap[degree]`` where given); J ~ U[-1, 1], h = 0; Trotter steps 100..300 growing with log n
+    (``instances.serving_steps``)."""
     rng = np.random.default_rng(seed)
     out, t = [], 0.0
     for k in range(count):
@@ -171,6 +298,8 @@ def mixed_requests(count: int, seed: int = 5, n_min: int = 100, n_max: int = 500
         if (n * d) % 2:
             n += 1
         chi = int(rng.choice(chis))
+        if chi_cap and d in chi_cap:
+            chi = min(chi, int(chi_cap[d]))
         nodes, edges = instances.regular_uniform(n, d, seed=int(rng.integers(1 << 30)), field_sigma=0.0)
         sched = instances.anneal_schedule(total_time, instances.serving_steps(n))
         sched["actions"][0]["type"] = "real_time_evolution"

@@ -217,6 +217,49 @@ int bqa_sv_get_state(bqa_sv* sv, double* out);
 int bqa_timing_enable(bqa_job* job, int32_t on);
 int bqa_timing_read(bqa_job* job, int64_t* launches, double* total_ms, double* bytes);
 
+/* Serving batch (no counterpart in the reference, which runs one job per process):
+ * one Trotter step of several jobs, each as bqa_run_layer would run it, with their
+ * phases on the batch's stream.  The extended messages and canonicalisers of all
+ * jobs run before one synchronisation for the truncation rule; each BP sweep is one
+ * launch per (precision, degree, chi) shape over the classes of every job that has
+ * it (k_small / k_mfma8 shapes; a job's other classes keep their own launches), with
+ * each job's convergence and iteration rules applied to its own status after every
+ * chunk of sweeps.  A job's results equal those of bqa_run_layer on it alone.  The
+ * jobs must be distinct, on the batch's device, with kernel timing off, and not in
+ * use by another thread during the call; per-job arrays have `count` entries.
+ * job_status[k] is BQA_ERR_NUMERIC where job k's BP gave no finite distance (the
+ * error bqa_run_layer returns for it; its state has advanced as there) and 0
+ * otherwise; any other failure fails the whole call. */
+typedef struct bqa_batch bqa_batch;
+int bqa_batch_create(bqa_batch** out, int32_t max_jobs);
+int bqa_batch_destroy(bqa_batch* batch);
+int bqa_batch_run_layer(bqa_batch* batch, int32_t count, bqa_job* const* jobs, const double* xtime,
+                        const double* ztime, const double* pinv_eps, const int32_t* max_iters, const double* bp_eps,
+                        const double* damping, int32_t* new_bond_dim, double* trunc_err, int32_t* sweeps,
+                        int32_t* converged, double* last_dist, int32_t* job_status);
+
+/* BP launch autotuner (no counterpart in the reference, which has one code path
+ * per operation).  Each (device, precision, degree, chi, ceil(log2 n)) shape is
+ * timed once per process, on the first sweep of a BP run that meets it, over its
+ * launch variants (wave splits of a node and grid policies); later launches use
+ * the fastest.  BQA_AUTOTUNE=0 keeps variant 0; BQA_BP_VARIANT=k forces variant k.
+ * bqa_tuning_report writes the table as JSON text into buf when capacity suffices
+ * and sets *needed to its size with the terminating zero; bqa_tuning_reset
+ * forgets every choice. */
+int bqa_tuning_report(char* buf, int64_t capacity, int64_t* needed);
+int bqa_tuning_reset(void);
+
+/* Graph-captured BP chunks (no counterpart in the reference, whose sweeps are
+ * host-driven library calls, state.py:105-121).  Once a job has run BP eight
+ * times in a row at one bond dimension, its sweeps run as replays of a HIP graph
+ * holding four sweeps of every degree class; a graph is captured once per bond dimension, buffer set and
+ * launch variants (thread-local capture mode) and replayed until the run
+ * converges.  Kernel timing (bqa_timing_enable) and BQA_GRAPHS=0 keep the
+ * launches eager, and so does a capture that a runtime call from outside this
+ * library invalidates (counted as a failure; the job continues eagerly).  Counts
+ * of captures, replays and failed captures so far: */
+int bqa_job_graph_stats(const bqa_job* job, int64_t* captures, int64_t* launches, int64_t* failures);
+
 /* Plain device-to-device copy of `bytes` bytes (HBM counter calibration). */
 int bqa_copy_calibrate(uint64_t bytes, int32_t repeats, double* ms_per_copy);
 

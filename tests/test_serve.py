@@ -85,14 +85,99 @@ def test_mixed_requests_are_seeded_and_valid():
     assert all(100 <= r.tag["n"] <= 301 and (r.tag["n"] * r.tag["degree"]) % 2 == 0 for r in a)
 
 
+class FakeJob:
+    def __init__(self, ctx, fail_bp=False):
+        self.ctx, self.fail_bp, self.closed = ctx, fail_bp, False
+
+    def density(self):
+        return None, np.zeros((self.ctx.nodes_number, 3))
+
+    def device_bytes(self):
+        return 100
+
+    def close(self):
+        self.closed = True
+
+
+class FakeBackend:
+    """Stand-in jobs and batch: records how many jobs each batched step held and the bytes resident."""
+
+    def __init__(self, footprint_by_n, fail_create=(), fail_bp=()):
+        self.fp, self.fail_create, self.fail_bp = footprint_by_n, set(fail_create), set(fail_bp)
+        self.jobs, self.widths, self.peak_bytes = [], [], 0
+        outer = self
+
+        class FakeBatch:
+            def __init__(self, max_jobs):
+                self.max_jobs, self.closed = max_jobs, False
+
+            def run_layer(self, jobs, xtime, ztime, pinv_eps, max_iters, bp_eps, damping):
+                assert len(jobs) <= self.max_jobs and len(set(map(id, jobs))) == len(jobs)
+                assert all(not j.closed for j in jobs)
+                outer.widths.append(len(jobs))
+                live = sum(outer.fp(j.ctx.nodes_number) for j in outer.jobs if not j.closed)
+                outer.peak_bytes = max(outer.peak_bytes, live)
+                time.sleep(0.002)
+                return [{"bond_dim": 1, "trunc_err": 0.0, "sweeps": 1, "converged": True, "dist": 0.0,
+                         "error": "BP produced no finite message distance" if j.fail_bp else None} for j in jobs]
+
+            def close(self):
+                self.closed = True
+
+        self.batch_cls = FakeBatch
+
+    def make_job(self, ctx, precision):
+        if ctx.nodes_number in self.fail_create:
+            raise MemoryError("planted allocation failure")
+        job = FakeJob(ctx, fail_bp=ctx.nodes_number in self.fail_bp)
+        self.jobs.append(job)
+        return job
+
+    def make_batch(self, max_jobs):
+        self.batch = self.batch_cls(max_jobs)
+        return self.batch
+
+
+def test_batched_executor_admission_and_failures():
+    """The batched executor steps at most max_concurrent jobs per call within the HBM budget, in
+    arrival order; a job whose creation or BP fails is reported and the others complete."""
+    reqs = [Request(_cfg(10 + 2 * k, k, steps=3 + k % 3), arrival=0.005 * k, tag=k) for k in range(10)]
+    fb = FakeBackend(lambda n: 1000 * n, fail_create={14}, fail_bp={20})
+    out = serve(reqs, max_concurrent=3, hbm_budget=70_000, footprint=lambda ctx, p: 1000 * ctx.nodes_number,
+                backend=fb)
+    assert out["jobs"] == 10 and out["failed"] == 2
+    errs = {o.tag: o.error for o in out["outcomes"] if o.error}
+    assert "planted allocation failure" in errs[2] and "no finite" in errs[5]
+    assert max(fb.widths) <= 3 and fb.peak_bytes <= 70_000
+    assert all(j.closed for j in fb.jobs) and fb.batch.closed
+    for r, o in zip(reqs, out["outcomes"]):
+        assert o.start >= o.arrival - 1e-9 and o.finish >= o.start
+        if o.error is None:
+            steps = r.config["schedule"]["actions"][0]["steps_number"]
+            assert o.steps == steps and len(o.results) == 1 and o.results[0][0] == "bloch_vectors"
+
+
+def _mix_cfg(n, d, chi, seed, steps):
+    nodes, edges = instances.regular_uniform(n, d, seed=seed, field_sigma=0.3)
+    return {"nodes": nodes, "edges": edges, "max_bond_dim": chi, "backend": "mi355x",
+            "schedule": instances.anneal_schedule(2.0, steps)}
+
+
 @pytest.mark.gpu
-def test_served_jobs_equal_jobs_alone():
+@pytest.mark.parametrize("precision", ["single", "double"])
+@pytest.mark.parametrize("batched", [True, False])
+def test_served_jobs_equal_jobs_alone(batched, precision):
+    """Jobs of degrees 3, 4 and 5 at chi 4 and 8 served together (batched: shared BP launches;
+    threaded: one stream each) give the Bloch vectors of each job run alone, bit for bit."""
     from bqa_amd.config import config_to_context
     from bqa_amd.serve import run_job
-    reqs = [Request(_cfg(n, 40 + n, chi=c, steps=8), arrival=0.0, tag=n) for n, c in ((24, 4), (40, 8), (60, 4), (30, 8))]
-    print("[gpu] serving 4 jobs together", flush=True)
-    out = serve(reqs, max_concurrent=4)
-    assert out["failed"] == 0
+    mix = ((24, 3, 4), (40, 3, 8), (60, 3, 4), (30, 4, 8), (36, 4, 4), (20, 5, 4), (26, 3, 8))
+    reqs = [Request(_mix_cfg(n, d, c, 40 + k, 10 + 2 * k), arrival=0.0, tag=(n, d, c))
+            for k, (n, d, c) in enumerate(mix)]
+    print(f"[gpu] serving {len(reqs)} jobs together, batched={batched}, {precision}", flush=True)
+    out = serve(reqs, max_concurrent=5, precision=precision, batched=batched)
+    assert out["failed"] == 0, [o.error for o in out["outcomes"]]
     for r, o in zip(reqs, out["outcomes"]):
-        alone, _, _ = run_job(config_to_context(dict(r.config)), "single")
-        assert np.array_equal(np.array(alone[0][1]), np.array(o.results[0][1]))
+        alone, steps, _ = run_job(config_to_context(dict(r.config)), precision)
+        assert o.steps == steps
+        assert np.array_equal(np.array(alone[0][1]), np.array(o.results[0][1])), r.tag
