@@ -58,7 +58,7 @@ BP_EPS = 1e-6
 BP_ITERS = 75
 PINV_EPS = 1e-6
 SERVE_SEED = 5
-SERVE_SLOTS = 16             # resident jobs of the batched serving executor
+SERVE_SLOTS = 32             # resident jobs of the batched serving executor (four batches of eight)
 SAMPLE_QUBITS = 100          # oracle sample: full 200-step anneal on the CPU (~30 s, one core)
 GRID_SIDE = 100
 GRID_STEPS = 10              # timed chi = 8 steps of the 100 x 100 grid line
@@ -269,6 +269,46 @@ def cpu_sample(chi, precisions):
     return cpu, parity
 
 
+def serving_sample(jobs, degrees, chi_cap):
+    """The serving figures for one job mix: offered load, capacity (all queued at t = 0) with the batched
+    executor and with one thread per job, and the same jobs one at a time."""
+    from bqa_amd.serve import mixed_requests, serve
+    log(f"[bench] serving: {jobs} mixed jobs, degrees {degrees}, Poisson arrivals")
+    reqs = mixed_requests(jobs, seed=SERVE_SEED, n_min=100, n_max=5000, rate_per_s=8.0, degrees=degrees,
+                          chis=(4, 8), chi_cap=chi_cap)
+    sv = serve(reqs, max_concurrent=SERVE_SLOTS)
+    keep = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "outcomes"}
+    pick = ("jobs", "failed", "makespan_s", "jobs_per_h", "qubit_steps_per_s", "p50_s", "p99_s")
+    out = keep(sv)
+    deg = ", ".join(str(d) for d in degrees)
+    cap_txt = "" if not chi_cap else " (" + ", ".join(f"chi {c} at d = {d}" for d, c in chi_cap.items()) + ")"
+    out["workload"] = (f"{jobs} jobs, Poisson arrivals at 8/s (seed {SERVE_SEED}); random d-regular with d in "
+                       f"{{{deg}}}, n log-uniform in [100, 5000], chi in {{4, 8}}{cap_txt}, J ~ U[-1,1], h = 0, "
+                       "T = 10, 100-300 Trotter steps growing with log n; FP32; at most "
+                       f"{SERVE_SLOTS} resident jobs, batched executor (four host threads, each stepping its "
+                       "resident jobs together through its own bqa_batch: canonicalisers and BP sweeps in launches "
+                       "shared per shape)")
+    out["note"] = ("jobs_per_h and p50/p99 at the offered load above; capacity: the same jobs all queued at t = 0; "
+                   "threaded: the same queue on one host thread and stream per job (8 resident); one_at_a_time: "
+                   "each job alone on its own bqa_job (no overlap, no sharing)")
+    out["mix"] = {f"d{d}_chi{c}": sum(1 for r in reqs if r.tag["degree"] == d and r.tag["chi"] == c)
+                  for d in degrees for c in (4, 8)}
+    for r in reqs:
+        r.arrival = 0.0
+    cap = serve(reqs, max_concurrent=SERVE_SLOTS)
+    out["capacity"] = {k: v for k, v in keep(cap).items() if k in pick}
+    thr = serve(reqs, max_concurrent=8, batched=False)
+    out["threaded"] = {k: v for k, v in keep(thr).items() if k in pick}
+    seq = serve(reqs, max_concurrent=1, batched=False)
+    out["one_at_a_time"] = {k: v for k, v in keep(seq).items() if k in pick}
+    out["capacity_over_one_at_a_time"] = round(cap["jobs_per_h"] / max(1e-12, seq["jobs_per_h"]), 3)
+    log(f"[bench] serving d in {degrees}: {out['jobs_per_h']:.4g} jobs/h offered, p50 {out['p50_s']:.3f} s, "
+        f"p99 {out['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h (threaded {thr['jobs_per_h']:.4g}); "
+        f"failed {out['failed']}; one at a time {seq['jobs_per_h']:.4g} jobs/h "
+        f"({out['capacity_over_one_at_a_time']:.2f}x)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -393,37 +433,8 @@ def main():
         extra["heavy_hex_chi16"] = hx
     serving = None
     if args.serve_jobs > 0:
-        from bqa_amd.serve import mixed_requests, serve
-        log(f"[bench] serving: {args.serve_jobs} mixed jobs, Poisson arrivals")
-        reqs = mixed_requests(args.serve_jobs, seed=SERVE_SEED, n_min=100, n_max=5000, rate_per_s=8.0,
-                              degrees=(3, 4, 5), chis=(4, 8), chi_cap={5: 4})
-        sv = serve(reqs, max_concurrent=SERVE_SLOTS)
-        keep = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "outcomes"}
-        pick = ("jobs", "failed", "makespan_s", "jobs_per_h", "qubit_steps_per_s", "p50_s", "p99_s")
-        serving = keep(sv)
-        serving["workload"] = (f"{args.serve_jobs} jobs, Poisson arrivals at 8/s (seed {SERVE_SEED}); random "
-                               "d-regular with d in {3, 4, 5}, n log-uniform in [100, 5000], chi in {4, 8} (chi 4 at "
-                               "d = 5), J ~ U[-1,1], h = 0, T = 10, 100-300 Trotter steps growing with log n; FP32; "
-                               f"at most {SERVE_SLOTS} resident jobs, batched executor (one bqa_batch step of all "
-                               "resident jobs per round, BP sweeps in launches shared per shape)")
-        serving["note"] = ("jobs_per_h and p50/p99 at the offered load above; capacity: the same jobs all queued "
-                           "at t = 0; one_at_a_time: each job alone on its own bqa_job (no overlap, no sharing); "
-                           "threaded: the same queue on one host thread and stream per job")
-        serving["mix"] = {f"d{d}_chi{c}": sum(1 for r in reqs if r.tag["degree"] == d and r.tag["chi"] == c)
-                          for d in (3, 4, 5) for c in (4, 8)}
-        for r in reqs:
-            r.arrival = 0.0
-        cap = serve(reqs, max_concurrent=SERVE_SLOTS)
-        serving["capacity"] = {k: v for k, v in keep(cap).items() if k in pick}
-        thr = serve(reqs, max_concurrent=8, batched=False)
-        serving["threaded"] = {k: v for k, v in keep(thr).items() if k in pick}
-        seq = serve(reqs, max_concurrent=1, batched=False)
-        serving["one_at_a_time"] = {k: v for k, v in keep(seq).items() if k in pick}
-        serving["capacity_over_one_at_a_time"] = round(cap["jobs_per_h"] / max(1e-12, seq["jobs_per_h"]), 3)
-        log(f"[bench] serving: {serving['jobs_per_h']:.4g} jobs/h offered, p50 {serving['p50_s']:.3f} s, "
-            f"p99 {serving['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h (threaded "
-            f"{thr['jobs_per_h']:.4g}); failed {serving['failed']}; one at a time {seq['jobs_per_h']:.4g} jobs/h "
-            f"({serving['capacity_over_one_at_a_time']:.2f}x)")
+        serving = serving_sample(args.serve_jobs, degrees=(3, 4, 5), chi_cap={5: 4})
+        serving["small_jobs_d3"] = serving_sample(args.serve_jobs, degrees=(3,), chi_cap=None)
     cpu, oracle_parity = None, None
     if not args.no_cpu:
         log(f"[bench] cpu baseline and oracle parity sample ({SAMPLE_QUBITS} qubits, {BASE_STEPS} steps)")

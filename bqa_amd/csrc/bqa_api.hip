@@ -849,9 +849,10 @@ void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64
 }
 
 void canonicalise(bqa_job* j, double pinv_eps, bool sync = true);
+void canon_outputs(bqa_job* j);
 
 template <typename R>
-void ext_and_canon(bqa_job* j, double ztime, double pinv_eps, bool sync = true) {
+void ext_and_canon(bqa_job* j, double ztime, double pinv_eps, bool sync = true, bool canon = true) {
   const int chi = j->chi, n2 = 2 * chi;
   j->ext.ensure((size_t)j->M * n2 * n2 * sizeof(cplx<double>), j->dev_bytes, j->stream);
   // the step's ZZ-split gates per edge, read by the extended-message and absorb kernels
@@ -883,20 +884,47 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps, bool sync = true) 
     if (c.d == 0 || c.n == 0) continue;
     ext_class<R>(j, c, ztime, msgs64);
   }
-  canonicalise(j, pinv_eps, sync);
+  if (canon) canonicalise(j, pinv_eps, sync);
 }
 
 // FP64 canonicalisers and lambdas of the 2chi x 2chi messages in j->ext (state.py:171-200), then
 // the per-column lambda maxima on the host (the global truncation rule reads them); without
 // `sync` the caller synchronises the stream before reading them.
+// The canonicaliser columns kept (the new bond dimension's bound) and the pseudo-inverse
+// threshold of the job's storage precision (backends.py:726).
+int canon_cols(const bqa_job* j) { return std::min(2 * j->chi, j->chimax); }
+double canon_deps(const bqa_job* j) {
+  return j->single() ? (double)std::numeric_limits<float>::epsilon() : std::numeric_limits<double>::epsilon();
+}
+
+// n of the split-column canonicaliser serving bond dimension chi, 0 if none does.
+int canon_h_n(int chi) { return chi >= 1 && chi <= 8 ? 2 * chi : chi == 16 ? 32 : 0; }
+
+void canon_outputs(bqa_job* j) {
+  const int n2 = 2 * j->chi;
+  j->canon.ensure((size_t)j->M * n2 * canon_cols(j) * sizeof(zd), j->dev_bytes, j->stream);
+  j->lmbd.ensure((size_t)j->E * n2 * sizeof(double), j->dev_bytes, j->stream);
+}
+
+// Per-column lambda maxima of the job's canonicalisers, copied to the host (read by the
+// truncation rule once the stream is synchronised).
+void lambda_readback(bqa_job* j) {
+  const int n2 = 2 * j->chi;
+  unsigned long long* lm = reinterpret_cast<unsigned long long*>(j->lmax.p);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, j->stream, lm, n2);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_lambda_max, dim3(grid_for((long)j->E * n2, 256, 1024)), dim3(256), 0, j->stream,
+                     reinterpret_cast<const double*>(j->lmbd.p), j->E, n2, lm);
+  LAUNCH_CHECK();
+  HIPX(hipMemcpyAsync(j->lmax_host, lm, sizeof(double) * n2, hipMemcpyDeviceToHost, j->stream));
+}
+
 void canonicalise(bqa_job* j, double pinv_eps, bool sync) {
   const int chi = j->chi, n2 = 2 * chi;
-  const int ccols = std::min(n2, j->chimax);
-  j->canon.ensure((size_t)j->M * n2 * ccols * sizeof(zd), j->dev_bytes, j->stream);
-  j->lmbd.ensure((size_t)j->E * n2 * sizeof(double), j->dev_bytes, j->stream);
+  const int ccols = canon_cols(j);
+  canon_outputs(j);
   const size_t wsb = canon_ws_bytes(n2);
-  const double deps = j->single() ? (double)std::numeric_limits<float>::epsilon()
-                                  : std::numeric_limits<double>::epsilon();
+  const double deps = canon_deps(j);
   if (j->generic(kGenCanon) || !canon_reg_dispatch(j, chi, ccols, pinv_eps, deps)) {
     int grid;
     size_t lds;
@@ -915,13 +943,7 @@ void canonicalise(bqa_job* j, double pinv_eps, bool sync) {
                        reinterpret_cast<zd*>(j->canon.p), reinterpret_cast<double*>(j->lmbd.p), ws, (long)wsb);
     LAUNCH_CHECK();
   }
-  unsigned long long* lm = reinterpret_cast<unsigned long long*>(j->lmax.p);
-  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, j->stream, lm, n2);
-  LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_lambda_max, dim3(grid_for((long)j->E * n2, 256, 1024)), dim3(256), 0, j->stream,
-                     reinterpret_cast<const double*>(j->lmbd.p), j->E, n2, lm);
-  LAUNCH_CHECK();
-  HIPX(hipMemcpyAsync(j->lmax_host, lm, sizeof(double) * n2, hipMemcpyDeviceToHost, j->stream));
+  lambda_readback(j);
   if (sync) HIPX(hipStreamSynchronize(j->stream));
 }
 
@@ -1314,6 +1336,12 @@ struct bqa_batch {
   int device = 0;
   int max_jobs = 0;
   hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> ready;     // per job slot: its simple update is done
+  std::vector<hipEvent_t> ext_done;  // per job slot: its extended messages are done
+  hipEvent_t canon_done = nullptr;   // the shared canonicaliser launches and readbacks are done
+  DevBuf ctab;                       // the shared canonicaliser launches' entry tables
+  CanonEntry* ctab_host = nullptr;
+  size_t ctab_cap = 0;
   DevBuf tab;                 // the BP launches' entry tables
   BpEntry* tab_host = nullptr;  // pinned staging of the tables
   size_t tab_cap = 0;         // entries both hold
@@ -1321,6 +1349,112 @@ struct bqa_batch {
 };
 
 namespace {
+
+// Split-column canonicaliser geometry of n: workspace doubles per edge, edges per 64-lane block.
+size_t canon_h_ws(int n) { return 10 * (size_t)n * n + 5 * (size_t)n; }
+int canon_h_epw(int n) {
+  const int p = n / 2, p2 = p <= 1 ? 1 : p <= 2 ? 2 : p <= 4 ? 4 : p <= 8 ? 8 : 16;
+  return 64 / (2 * p2);
+}
+
+template <int N>
+void launch_canon_multi(const CanonEntry* tab, int entries, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL((k_canon_h_multi<N>), dim3(blocks), dim3(64), 0, s, tab, entries);
+}
+
+bool canon_multi_dispatch(int n, const CanonEntry* tab, int entries, int blocks, hipStream_t s) {
+  switch (n) {
+    case 2: launch_canon_multi<2>(tab, entries, blocks, s); return true;
+    case 4: launch_canon_multi<4>(tab, entries, blocks, s); return true;
+    case 6: launch_canon_multi<6>(tab, entries, blocks, s); return true;
+    case 8: launch_canon_multi<8>(tab, entries, blocks, s); return true;
+    case 10: launch_canon_multi<10>(tab, entries, blocks, s); return true;
+    case 12: launch_canon_multi<12>(tab, entries, blocks, s); return true;
+    case 14: launch_canon_multi<14>(tab, entries, blocks, s); return true;
+    case 16: launch_canon_multi<16>(tab, entries, blocks, s); return true;
+    case 32: launch_canon_multi<32>(tab, entries, blocks, s); return true;
+    default: return false;
+  }
+}
+
+// The canonicalisers of the batch's jobs whose bond dimension has a split-column kernel (state.py:171-200
+// per job): once each job's extended messages are done (its own stream), one launch per n over all of
+// them on the batch stream, then each job's lambda maxima readback there; each job's stream then waits
+// for that.  Jobs marked false in `shared` ran their own canonicaliser with their extended messages.
+void batch_canon(bqa_batch* b, int count, bqa_job* const* jobs, const double* pinv_eps,
+                 const std::vector<char>& shared) {
+  std::vector<int> order;
+  for (int k = 0; k < count; ++k)
+    if (shared[k]) order.push_back(k);
+  if (order.empty()) return;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return canon_h_n(jobs[x]->chi) < canon_h_n(jobs[y]->chi); });
+  for (int k : order) {
+    bqa_job* j = jobs[k];
+    canon_outputs(j);
+    j->cws.ensure((size_t)j->E * canon_h_ws(canon_h_n(j->chi)) * sizeof(double), j->dev_bytes, j->stream);
+    HIPX(hipEventRecord(b->ext_done[k], j->stream));
+    HIPX(hipStreamWaitEvent(b->stream, b->ext_done[k], 0));
+  }
+  const size_t entries = order.size();
+  if (entries > b->ctab_cap) {
+    HIPX(hipStreamSynchronize(b->stream));
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    if (b->ctab_host) (void)hipHostFree(b->ctab_host);
+    b->ctab_host = nullptr;
+    b->ctab_cap = 0;
+    HIPX(hipHostMalloc(reinterpret_cast<void**>(&b->ctab_host), entries * sizeof(CanonEntry), hipHostMallocDefault));
+    b->ctab_cap = entries;
+  }
+  b->ctab.ensure(entries * sizeof(CanonEntry), b->dev_bytes, b->stream);
+  struct Launch {
+    int n;
+    size_t first, count;
+    int blocks;
+  };
+  std::vector<Launch> launches;
+  for (size_t x = 0; x < entries; ++x) {
+    bqa_job* j = jobs[order[x]];
+    const int n = canon_h_n(j->chi);
+    if (launches.empty() || launches.back().n != n) launches.push_back({n, x, 0, 0});
+    Launch& L = launches.back();
+    CanonEntry& e = b->ctab_host[x];
+    e.ext = reinterpret_cast<const cplx<double>*>(j->ext.p);
+    e.canon = reinterpret_cast<zd*>(j->canon.p);
+    e.lfull = reinterpret_cast<double*>(j->lmbd.p);
+    e.ws = reinterpret_cast<double*>(j->cws.p);
+    e.E = j->E;
+    e.ccols = canon_cols(j);
+    e.pinv_eps = pinv_eps[order[x]];
+    e.deps = canon_deps(j);
+    e.block0 = L.blocks;
+    const long blocks = (long)L.blocks + (j->E + canon_h_epw(n) - 1) / canon_h_epw(n);
+    require(blocks <= std::numeric_limits<int>::max(), "too many edges in one shared canonicaliser launch");
+    L.blocks = (int)blocks;
+    ++L.count;
+  }
+  HIPX(hipMemcpyAsync(b->ctab.p, b->ctab_host, entries * sizeof(CanonEntry), hipMemcpyHostToDevice, b->stream));
+  const CanonEntry* tab = reinterpret_cast<const CanonEntry*>(b->ctab.p);
+  for (const auto& L : launches) {
+    if (!canon_multi_dispatch(L.n, tab + L.first, (int)L.count, L.blocks, b->stream))
+      throw BqaError(BQA_ERR_HIP, "no shared canonicaliser for n = " + std::to_string(L.n));
+    LAUNCH_CHECK();
+  }
+  for (int k : order) {
+    bqa_job* j = jobs[k];
+    const hipStream_t own = j->stream;
+    j->stream = b->stream;
+    try {
+      lambda_readback(j);
+    } catch (...) {
+      j->stream = own;
+      throw;
+    }
+    j->stream = own;
+  }
+  HIPX(hipEventRecord(b->canon_done, b->stream));
+  for (int k : order) HIPX(hipStreamWaitEvent(jobs[k]->stream, b->canon_done, 0));
+}
 
 // BP of the batch's jobs (src/bqa/state.py:97-124 per job): every job's run starts together on the
 // batch stream; each sweep is one shared launch per (precision, kernel family, degree, chi) over the
@@ -1955,8 +2089,15 @@ int bqa_batch_create(bqa_batch** out, int32_t max_jobs) {
     b = new bqa_batch();
     b->max_jobs = max_jobs;
     HIPX(hipGetDevice(&b->device));
-    std::lock_guard<std::mutex> g(g_capture_mu);
-    HIPX(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    {
+      std::lock_guard<std::mutex> g(g_capture_mu);
+      HIPX(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    }
+    b->ready.assign(max_jobs, nullptr);
+    b->ext_done.assign(max_jobs, nullptr);
+    for (auto& e : b->ready) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : b->ext_done) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPX(hipEventCreateWithFlags(&b->canon_done, hipEventDisableTiming));
     *out = b;
   });
   if (rc != BQA_OK && b) bqa_batch_destroy(b);
@@ -1968,10 +2109,17 @@ int bqa_batch_destroy(bqa_batch* b) {
   return guarded([&] {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto e : b->ready)
+      if (e) (void)hipEventDestroy(e);
+    for (auto e : b->ext_done)
+      if (e) (void)hipEventDestroy(e);
+    if (b->canon_done) (void)hipEventDestroy(b->canon_done);
     b->tab.release(b->dev_bytes);
+    b->ctab.release(b->dev_bytes);
     {
       std::lock_guard<std::mutex> g(g_capture_mu);
       if (b->tab_host) (void)hipHostFree(b->tab_host);
+      if (b->ctab_host) (void)hipHostFree(b->ctab_host);
       if (b->stream) (void)hipStreamDestroy(b->stream);
     }
     delete b;
@@ -1998,28 +2146,37 @@ int bqa_batch_run_layer(bqa_batch* b, int32_t count, bqa_job* const* jobs, const
       for (int q = 0; q < k; ++q) require(jobs[q] != jobs[k], "a job appears twice in the batch");
     }
     HIPX(hipSetDevice(b->device));
-    // every job's earlier work is complete before its phases move to the batch stream
+    // the simple update of every job on the job's own stream, so small jobs' kernels overlap:
+    // extended messages of all jobs; their canonicalisers in launches shared per n (a job without a
+    // split-column kernel for its chi runs its own); the lambda maxima readbacks
+    std::vector<char> shared(count);
+    for (int k = 0; k < count; ++k)
+      shared[k] = !jobs[k]->generic(kGenCanon) && jobs[k]->E > 0 && canon_h_n(jobs[k]->chi) > 0;
+    for (int k = 0; k < count; ++k)
+      dispatch(jobs[k], [&](auto r) {
+        ext_and_canon<decltype(r)>(jobs[k], ztime[k], pinv_eps[k], false, !shared[k]);
+      });
+    batch_canon(b, count, jobs, pinv_eps, shared);
     for (int k = 0; k < count; ++k) HIPX(hipStreamSynchronize(jobs[k]->stream));
+    // the global truncation rule of each job, then its fused absorb + gates + gauge step
+    for (int k = 0; k < count; ++k)
+      dispatch(jobs[k], [&](auto r) {
+        bqa_job* j = jobs[k];
+        const int n2 = 2 * j->chi;
+        const int dim = truncation_dim(j, n2, pinv_eps[k], &trunc_err[k]);
+        j->lstride = n2;
+        local_update<decltype(r)>(j, kAbsorb | kGates | kGauge, dim, ztime[k], ztime[k], xtime[k]);
+        new_bond_dim[k] = dim;
+        HIPX(hipEventRecord(b->ready[k], j->stream));
+        HIPX(hipStreamWaitEvent(b->stream, b->ready[k], 0));
+      });
+    // BP on the batch stream: shared launches and every job's own BP launches in sweep order
     std::vector<hipStream_t> own(count);
     for (int k = 0; k < count; ++k) {
       own[k] = jobs[k]->stream;
       jobs[k]->stream = b->stream;
     }
     try {
-      // extended messages, canonicalisers and lambda maxima of every job; one synchronisation
-      for (int k = 0; k < count; ++k)
-        dispatch(jobs[k], [&](auto r) { ext_and_canon<decltype(r)>(jobs[k], ztime[k], pinv_eps[k], false); });
-      HIPX(hipStreamSynchronize(b->stream));
-      // the global truncation rule of each job, then its fused absorb + gates + gauge step
-      for (int k = 0; k < count; ++k)
-        dispatch(jobs[k], [&](auto r) {
-          bqa_job* j = jobs[k];
-          const int n2 = 2 * j->chi;
-          const int dim = truncation_dim(j, n2, pinv_eps[k], &trunc_err[k]);
-          j->lstride = n2;
-          local_update<decltype(r)>(j, kAbsorb | kGates | kGauge, dim, ztime[k], ztime[k], xtime[k]);
-          new_bond_dim[k] = dim;
-        });
       batch_bp(b, count, jobs, max_iters, bp_eps, damping, sweeps, converged, last_dist, job_status);
     } catch (...) {
       (void)hipStreamSynchronize(b->stream);

@@ -337,17 +337,18 @@ __device__ void hcholesky(HCol<N / 2>& top, HCol<N / 2>& bot, int k, int h, bool
   }
 }
 
+// The edges of block `blk` of one job's launch: blk * EPW .. blk * EPW + EPW - 1.
 template <int N>
-__global__ __launch_bounds__(64, N <= 16 ? 3 : 1) void k_canon_h(const cplx<double>* __restrict__ ext, int E, int ccols,
-                                                double pinv_eps, double deps, zd* __restrict__ canon,
-                                                double* __restrict__ lfull, double* __restrict__ ws) {
+__device__ __forceinline__ void canon_h_run(const cplx<double>* __restrict__ ext, int E, int ccols, double pinv_eps,
+                                            double deps, zd* __restrict__ canon, double* __restrict__ lfull,
+                                            double* __restrict__ ws, int blk) {
   using S = HCanon<N>;
   constexpr int P = S::P, P2 = S::P2, G = S::G, NH = S::NH;
   const int lane = threadIdx.x;
   const int w = lane % G;
   const int base = lane - w;
   const int h = w / P2, k = w % P2;
-  const int edge = blockIdx.x * S::EPW + lane / G;
+  const int edge = blk * S::EPW + lane / G;
   const bool valid = edge < E;
   const bool active = valid && k < P;
   double* wsE = ws + (size_t)(valid ? edge : 0) * S::WS;
@@ -685,6 +686,42 @@ __global__ __launch_bounds__(64, N <= 16 ? 3 : 1) void k_canon_h(const cplx<doub
       Cb[x] = mk(br_, bi_);
     }
   }
+}
+
+template <int N>
+__global__ __launch_bounds__(64, N <= 16 ? 3 : 1) void k_canon_h(const cplx<double>* __restrict__ ext, int E, int ccols,
+                                                double pinv_eps, double deps, zd* __restrict__ canon,
+                                                double* __restrict__ lfull, double* __restrict__ ws) {
+  canon_h_run<N>(ext, E, ccols, pinv_eps, deps, canon, lfull, ws, blockIdx.x);
+}
+
+// One job's edges in a launch that covers several jobs' canonicalisers (the serving batch):
+// its buffers and parameters, and its first block in the launch.
+struct CanonEntry {
+  const cplx<double>* ext;
+  zd* canon;
+  double* lfull;
+  double* ws;
+  int E, ccols;
+  double pinv_eps, deps;
+  int block0;
+};
+
+// The canonicalisers of several jobs (same n) in one launch: entry e takes blocks
+// block0_e .. block0_{e+1} - 1 (found by binary search; block0 ascending, block0_0 = 0).
+template <int N>
+__global__ __launch_bounds__(64, N <= 16 ? 3 : 1) void k_canon_h_multi(const CanonEntry* __restrict__ tab,
+                                                                      int entries) {
+  int lo = 0, hi = entries - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (tab[mid].block0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const CanonEntry& e = tab[lo];
+  canon_h_run<N>(e.ext, e.E, e.ccols, e.pinv_eps, e.deps, e.canon, e.lfull, e.ws, (int)blockIdx.x - e.block0);
 }
 
 }  // namespace bqa

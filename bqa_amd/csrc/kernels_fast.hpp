@@ -30,7 +30,7 @@ int bp_variant_count(int d, int chi);
 template <typename R>
 inline bool bp_fast_shape(int d, int chi) {
   if ((chi == 8 || (sizeof(R) == sizeof(float) && chi == 16)) && (d == 2 || d == 3)) return true;
-  return small_shape(d, chi);
+  return small_bp_shape<R>(d, chi);
 }
 
 template <typename R>
@@ -102,7 +102,7 @@ template <>
 inline int bp_variant_count<float>(int d, int chi) {
   if (chi == 16 && (d == 2 || d == 3)) return 2;
   if (chi == 8 && (d == 2 || d == 3)) return kGridPolicies;
-  return small_shape(d, chi) ? kGridPolicies : 1;
+  return small_bp_shape<float>(d, chi) ? kGridPolicies : 1;
 }
 
 template <>
@@ -113,7 +113,7 @@ inline const char* bp_variant_name<float>(int d, int chi, int v) {
                                           "k_mfma8, 2x resident"};
   static const char* sm[kGridPolicies] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident"};
   if (chi == 8 && (d == 2 || d == 3)) return m8[v % kGridPolicies];
-  return small_shape(d, chi) ? sm[v % kGridPolicies] : "generic";
+  return small_bp_shape<float>(d, chi) ? sm[v % kGridPolicies] : "generic";
 }
 
 // FP32, chi = 8, degree 2 or 3: k_mfma8, one wave per node (two and four waves per node for
@@ -200,7 +200,7 @@ template <typename R>
 inline int multi_family(int d, int chi) {
   if (sizeof(R) == sizeof(float) && chi == 8 && (d == 2 || d == 3)) return 2;
   if (sizeof(R) == sizeof(double) && chi == 8 && (d == 2 || d == 3)) return 0;  // FP64 MFMA kernels
-  return small_shape(d, chi) ? 1 : 0;
+  return small_bp_shape<R>(d, chi) ? 1 : 0;
 }
 
 template <typename R, int D, int CHI>
@@ -251,6 +251,9 @@ inline bool launch_bp_multi(int fam, int d, int chi, const BpEntry* tab, int ent
     case 2: return launch_small_multi_d<R, 2>(chi, tab, entries, total, sweep, s);
     case 3: return launch_small_multi_d<R, 3>(chi, tab, entries, total, sweep, s);
     case 4: return launch_small_multi_d<R, 4>(chi, tab, entries, total, sweep, s);
+    case 5:
+      if constexpr (sizeof(R) == sizeof(float)) return launch_small_multi_d<R, 5>(chi, tab, entries, total, sweep, s);
+      return false;
     default: return false;
   }
 }

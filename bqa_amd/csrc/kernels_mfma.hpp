@@ -462,7 +462,7 @@ __device__ __forceinline__ void mf8_run(const ClassView& cv, const cplx<float>* 
     // the next node's tensor load overwrites T, which the last Gram read
     node_sync<W>();
   }
-  if (MODE == kModeBP && lead) {
+  if (MODE == kModeBP && lead && start < n) {  // a block without nodes here adds nothing to the maxima
     for (int o = 32; o > 0; o >>= 1) {
       dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
       dden = fmax(dden, __shfl_xor(dden, o, 64));
@@ -491,6 +491,7 @@ __global__ __launch_bounds__(64) void k_mfma8_multi(const BpEntry* __restrict__ 
     cv.sweep = sweep;
     const long g = gridDim.x;
     const int start = (int)((((long)blockIdx.x - tab[e].prefix) % g + g) % g);
+    if (start >= cv.n) continue;  // none of this entry's nodes falls to this block
     mf8_run<D, kModeBP, 1>(cv, reinterpret_cast<const cplx<float>*>(tab[e].T),
                            reinterpret_cast<cplx<float>*>(tab[e].mA), reinterpret_cast<cplx<float>*>(tab[e].mB),
                            tab[e].st, nullptr, nullptr, start);

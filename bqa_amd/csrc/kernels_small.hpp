@@ -205,7 +205,8 @@ template <int D, int CHI, int MODE, typename R, typename C>
 __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
                                           cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out, double* rho_out,
                                           double* bloch_out, int start) {
-  static_assert(D >= 2 && D <= 4 && CHI >= 2 && CHI <= 8, "shape");
+  static_assert(D >= 2 && D <= 5 && CHI >= 2 && CHI <= 8, "shape");
+  static_assert(D <= 4 || MODE == kModeBP, "degree 5: BP sweeps only");
   using M = Sm<D, CHI>;
   __shared__ __align__(16) SmallSmem<D, CHI, C> sm;
   const int n = cv.n;
@@ -311,7 +312,7 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y2 = W x1 m1
       small_sync();
       sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
-    } else {
+    } else if constexpr (D == 4) {
       // degree 4: W = T x3 m3 x2 m2 shared by outputs 0 and 1, V = T x0 m0 x1 m1
       // shared by outputs 2 and 3 (8 mode products instead of 12).
       smp<D, CHI, 3, C>(sm.T, sm.X, sm.m[3]);
@@ -334,9 +335,45 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
       smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);  // Y3 = V x2 m2
       small_sync();
       sout<D, CHI, 3, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+    } else {
+      // degree 5: A = T x2 m2 x3 m3 x4 m4 shared by outputs 0 and 1, B = T x0 m0 x1 m1 x2 m2
+      // shared by outputs 3 and 4, output 2 on its own (14 mode products instead of 20).
+      smp<D, CHI, 2, C>(sm.T, sm.X, sm.m[2]);
+      small_sync();
+      smp<D, CHI, 3, C>(sm.X, sm.Y, sm.m[3]);
+      small_sync();
+      smp<D, CHI, 4, C>(sm.Y, sm.X, sm.m[4]);  // A in X
+      small_sync();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y0 = A x1 m1
+      small_sync();
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      smp<D, CHI, 0, C>(sm.X, sm.Y, sm.m[0]);  // Y1 = A x0 m0
+      small_sync();
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
+      small_sync();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);
+      small_sync();
+      smp<D, CHI, 3, C>(sm.Y, sm.X, sm.m[3]);
+      small_sync();
+      smp<D, CHI, 4, C>(sm.X, sm.Y, sm.m[4]);  // Y2 = T x0 m0 x1 m1 x3 m3 x4 m4
+      small_sync();
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
+      small_sync();
+      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);
+      small_sync();
+      smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);  // B in X
+      small_sync();
+      smp<D, CHI, 4, C>(sm.X, sm.Y, sm.m[4]);  // Y3 = B x4 m4
+      small_sync();
+      sout<D, CHI, 3, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      smp<D, CHI, 3, C>(sm.X, sm.Y, sm.m[3]);  // Y4 = B x3 m3
+      small_sync();
+      sout<D, CHI, 4, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
     }
   }
-  if (MODE == kModeBP) {
+  if (MODE == kModeBP && start < n) {  // a block without nodes here adds nothing to the maxima
     for (int o = 32; o > 0; o >>= 1) {
       dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
       dden = fmax(dden, __shfl_xor(dden, o, 64));
@@ -365,6 +402,7 @@ __global__ __launch_bounds__(64) void k_small_multi(const BpEntry* __restrict__ 
     cv.sweep = sweep;
     const long g = gridDim.x;
     const int start = (int)((((long)blockIdx.x - tab[e].prefix) % g + g) % g);
+    if (start >= cv.n) continue;  // none of this entry's nodes falls to this block
     small_run<D, CHI, kModeBP, R, R>(cv, reinterpret_cast<const cplx<R>*>(tab[e].T),
                                      reinterpret_cast<cplx<R>*>(tab[e].mA), reinterpret_cast<cplx<R>*>(tab[e].mB),
                                      tab[e].st, nullptr, nullptr, nullptr, start);
@@ -374,6 +412,13 @@ __global__ __launch_bounds__(64) void k_small_multi(const BpEntry* __restrict__ 
 // Shapes served: degree 2 and 3 with chi 2..8, degree 4 with chi 2..4.
 inline bool small_shape(int d, int chi) {
   return chi >= 2 && ((d == 2 || d == 3) ? chi <= 8 : (d == 4 && chi <= 4));
+}
+
+// BP sweeps also serve degree 5 with chi 2..4 for FP32 storage (three 16 KB complex64 node
+// buffers at chi = 4; FP64 would need 96 KB of LDS per one-wave block).
+template <typename R>
+inline bool small_bp_shape(int d, int chi) {
+  return small_shape(d, chi) || (sizeof(R) == sizeof(float) && d == 5 && chi >= 2 && chi <= 4);
 }
 
 inline int small_grid(int n) { return n < 8192 ? n : 8192; }
@@ -418,7 +463,15 @@ inline bool launch_small_d(int chi, ClassView cv, const cplx<R>* T, cplx<R>* mA,
 template <int MODE, typename R, typename C>
 inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
                          cplx<double>* ext, double* rho, double* bloch, hipStream_t s, int policy = 0) {
-  if (cv.n == 0 || !small_shape(cv.d, chi)) return false;
+  if (cv.n == 0) return false;
+  if (cv.d == 5) {
+    if constexpr (MODE == kModeBP && sizeof(C) == sizeof(float) && sizeof(R) == sizeof(float)) {
+      if (small_bp_shape<R>(cv.d, chi))
+        return launch_small_d<5, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
+    }
+    return false;
+  }
+  if (!small_shape(cv.d, chi)) return false;
   switch (cv.d) {
     case 2: return launch_small_d<2, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
     case 3: return launch_small_d<3, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);

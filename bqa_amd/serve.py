@@ -5,11 +5,11 @@ Jobs are admitted in arrival order once they have arrived, a slot is free
 (at most ``max_concurrent`` jobs resident) and their device footprint
 (``bqa_job_estimate_bytes``) fits the remaining HBM budget.  Two executors:
 
-* batched (the default): one host thread steps every resident job together
-  through a ``bqa_batch`` -- each Trotter step of all resident jobs is one call,
-  with the jobs' canonicalisers before one synchronisation and their BP sweeps
-  in launches shared per (precision, degree, chi) shape; a job joins the batch
-  at its next step once admitted and leaves it after its last instruction;
+* batched (the default): a few host threads, each stepping its resident jobs
+  together through its own ``bqa_batch`` -- one call per Trotter step of all of
+  them, with their canonicalisers and BP sweeps in launches shared per shape; a
+  job joins a batch at its next step once admitted and leaves it after its last
+  instruction;
 * threaded: every job runs its instruction stream on its own host thread with
   its own ``bqa_job`` handle and HIP stream, so the sweeps of concurrent jobs
   overlap on the GPU.
@@ -106,7 +106,7 @@ class NativeBackend:
 
 
 def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGET, precision: str = "single",
-          runner=None, footprint=None, batched: bool = True, backend=None) -> dict:
+          runner=None, footprint=None, batched: bool = True, backend=None, batch_workers: int = 4) -> dict:
     """Run the requests through the admission loop; see the module docstring.
 
     ``footprint(ctx, precision) -> bytes`` defaults to ``bqa_job_estimate_bytes``.  The
@@ -133,7 +133,8 @@ def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGE
             raise MemoryError(f"request {req.tag!r} needs {need} bytes, more than the HBM budget {hbm_budget}")
         prepared.append((idx, req, ctx, need))
     if batched:
-        outs = _serve_batched(prepared, max_concurrent, hbm_budget, precision, backend or NativeBackend)
+        outs = _serve_batched(prepared, max_concurrent, hbm_budget, precision, backend or NativeBackend,
+                              workers=batch_workers)
         return _summary(outs, max_concurrent)
     outcomes: dict[int, Outcome] = {}
     lock = threading.Condition()
@@ -176,92 +177,121 @@ def serve(requests, max_concurrent: int = 8, hbm_budget: int = DEFAULT_HBM_BUDGE
     return _summary([outcomes[i] for i in sorted(outcomes)], max_concurrent)
 
 
-def _serve_batched(prepared, max_concurrent, hbm_budget, precision, backend) -> list:
-    """The batched executor: admission as in the threaded one, then every resident job's next Trotter
-    step in one ``Batch.run_layer`` call per round; readouts between steps run on the job itself."""
-    batch = backend.make_batch(max_concurrent)
-    outcomes: dict[int, Outcome] = {}
+def _serve_batched(prepared, max_concurrent, hbm_budget, precision, backend, workers: int = 4) -> list:
+    """The batched executor: ``workers`` host threads, each stepping its own resident jobs together through
+    its own ``Batch`` (one ``Batch.run_layer`` call per round; readouts between steps run on the job itself),
+    so one batch's kernels run while another waits on its latency-bound phases.  Admission (arrival order,
+    the slots, the shared HBM budget) is common to all of them."""
+    workers = max(1, min(workers, max_concurrent))
+    caps = [max_concurrent // workers + (1 if w < max_concurrent % workers else 0) for w in range(workers)]
+    lock = threading.Lock()
     queue = list(prepared)
-    resident = []
-    used = 0
+    outcomes: dict[int, Outcome] = {}
+    used = [0]
     t0 = time.perf_counter()
+    errors = []
 
-    def finish(r, err):
-        nonlocal used
-        dbytes = 0
-        try:
-            dbytes = r["job"].device_bytes()
-        except Exception:
-            pass
-        try:
-            r["job"].close()
-        except Exception as e:  # reported with the job
-            err = err or f"{type(e).__name__}: {e}"
-        fin = time.perf_counter() - t0
-        req, ctx = r["req"], r["ctx"]
-        outcomes[r["idx"]] = Outcome(req.tag, r["results"] if err is None else [], req.arrival, r["start"], fin,
-                                     ctx.nodes_number, r["steps"], dbytes, err)
-        used -= r["need"]
-        resident.remove(r)
+    def record(idx, outcome, need):
+        with lock:
+            outcomes[idx] = outcome
+            used[0] -= need
 
-    try:
-        while queue or resident:
-            now = time.perf_counter() - t0
-            while queue and queue[0][1].arrival <= now and len(resident) < max_concurrent and \
-                    used + queue[0][3] <= hbm_budget:
-                idx, req, ctx, need = queue.pop(0)
-                try:
-                    job = backend.make_job(ctx, precision)
-                except Exception as e:  # recorded per job; the loop keeps serving the others
-                    outcomes[idx] = Outcome(req.tag, [], req.arrival, now, time.perf_counter() - t0,
-                                            ctx.nodes_number, 0, 0, f"{type(e).__name__}: {e}")
-                    continue
-                resident.append({"idx": idx, "req": req, "ctx": ctx, "need": need, "start": now, "job": job,
-                                 "pc": 0, "steps": 0, "results": []})
-                used += need
-            if not resident:
-                if queue:
-                    time.sleep(max(0.0, min(1.0, queue[0][1].arrival - (time.perf_counter() - t0))))
-                continue
-            stepping = []
-            for r in list(resident):
-                ins = r["ctx"].instructions
-                try:
-                    while r["pc"] < len(ins) and not isinstance(ins[r["pc"]], dict):
-                        if ins[r["pc"]] != "get_bloch_vectors":
-                            raise NotImplementedError(f"instruction {ins[r['pc']]!r} is not served")
-                        r["results"].append(["bloch_vectors", r["job"].density()[1].tolist()])
-                        r["pc"] += 1
-                except Exception as e:
-                    finish(r, f"{type(e).__name__}: {e}")
-                    continue
-                if r["pc"] >= len(ins):
-                    finish(r, None)
-                else:
-                    stepping.append(r)
-            if not stepping:
-                continue
-            lays = [r["ctx"].instructions[r["pc"]] for r in stepping]
-            cx = [r["ctx"] for r in stepping]
+    def worker(cap):
+        batch = backend.make_batch(cap)
+        resident = []
+
+        def finish(r, err):
+            dbytes = 0
             try:
-                outs = batch.run_layer([r["job"] for r in stepping], [l["xtime"] for l in lays],
-                                       [l["ztime"] for l in lays], [c.pinv_eps for c in cx],
-                                       [c.max_bp_iters_number for c in cx], [c.bp_eps for c in cx],
-                                       [c.damping for c in cx])
-            except Exception as e:  # the whole step failed: every job in it is reported
-                for r in stepping:
-                    finish(r, f"{type(e).__name__}: {e}")
-                continue
-            for r, o in zip(stepping, outs):
-                if o.get("error"):
-                    finish(r, f"AssertionError: {o['error']}")
-                else:
-                    r["pc"] += 1
-                    r["steps"] += 1
-    finally:
-        for r in list(resident):
-            finish(r, "RuntimeError: the serving loop stopped")
-        batch.close()
+                dbytes = r["job"].device_bytes()
+            except Exception:
+                pass
+            try:
+                r["job"].close()
+            except Exception as e:  # reported with the job
+                err = err or f"{type(e).__name__}: {e}"
+            req, ctx = r["req"], r["ctx"]
+            record(r["idx"], Outcome(req.tag, r["results"] if err is None else [], req.arrival, r["start"],
+                                     time.perf_counter() - t0, ctx.nodes_number, r["steps"], dbytes, err), r["need"])
+            resident.remove(r)
+
+        try:
+            while True:
+                now = time.perf_counter() - t0
+                taken, wait = [], None
+                with lock:
+                    while queue and queue[0][1].arrival <= now and len(resident) + len(taken) < cap and \
+                            used[0] + queue[0][3] <= hbm_budget:
+                        item = queue.pop(0)
+                        used[0] += item[3]
+                        taken.append(item)
+                    if not queue and not resident and not taken:
+                        break
+                    if queue and queue[0][1].arrival > now:
+                        wait = queue[0][1].arrival - now
+                for idx, req, ctx, need in taken:
+                    try:
+                        job = backend.make_job(ctx, precision)
+                    except Exception as e:  # recorded per job; the loop keeps serving the others
+                        record(idx, Outcome(req.tag, [], req.arrival, now, time.perf_counter() - t0,
+                                            ctx.nodes_number, 0, 0, f"{type(e).__name__}: {e}"), need)
+                        continue
+                    resident.append({"idx": idx, "req": req, "ctx": ctx, "need": need, "start": now, "job": job,
+                                     "pc": 0, "steps": 0, "results": []})
+                if not resident:
+                    time.sleep(min(0.01, wait) if wait is not None else 0.002)
+                    continue
+                stepping = []
+                for r in list(resident):
+                    ins = r["ctx"].instructions
+                    try:
+                        while r["pc"] < len(ins) and not isinstance(ins[r["pc"]], dict):
+                            if ins[r["pc"]] != "get_bloch_vectors":
+                                raise NotImplementedError(f"instruction {ins[r['pc']]!r} is not served")
+                            r["results"].append(["bloch_vectors", r["job"].density()[1].tolist()])
+                            r["pc"] += 1
+                    except Exception as e:
+                        finish(r, f"{type(e).__name__}: {e}")
+                        continue
+                    if r["pc"] >= len(ins):
+                        finish(r, None)
+                    else:
+                        stepping.append(r)
+                if not stepping:
+                    continue
+                lays = [r["ctx"].instructions[r["pc"]] for r in stepping]
+                cx = [r["ctx"] for r in stepping]
+                try:
+                    outs = batch.run_layer([r["job"] for r in stepping], [l["xtime"] for l in lays],
+                                           [l["ztime"] for l in lays], [c.pinv_eps for c in cx],
+                                           [c.max_bp_iters_number for c in cx], [c.bp_eps for c in cx],
+                                           [c.damping for c in cx])
+                except Exception as e:  # the whole step failed: every job in it is reported
+                    for r in stepping:
+                        finish(r, f"{type(e).__name__}: {e}")
+                    continue
+                for r, o in zip(stepping, outs):
+                    if o.get("error"):
+                        finish(r, f"AssertionError: {o['error']}")
+                    else:
+                        r["pc"] += 1
+                        r["steps"] += 1
+        except Exception as e:  # an executor failure: its jobs are reported below
+            errors.append(f"{type(e).__name__}: {e}")
+        finally:
+            for r in list(resident):
+                finish(r, errors[-1] if errors else "RuntimeError: the serving loop stopped")
+            batch.close()
+
+    threads = [threading.Thread(target=worker, args=(c,), daemon=True) for c in caps]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    with lock:
+        for idx, req, ctx, need in queue:  # only left if every executor failed
+            outcomes[idx] = Outcome(req.tag, [], req.arrival, 0.0, time.perf_counter() - t0, ctx.nodes_number, 0, 0,
+                                    errors[-1] if errors else "RuntimeError: not served")
     return [outcomes[i] for i in sorted(outcomes)]
 
 

@@ -218,9 +218,10 @@ int bqa_timing_enable(bqa_job* job, int32_t on);
 int bqa_timing_read(bqa_job* job, int64_t* launches, double* total_ms, double* bytes);
 
 /* Serving batch (no counterpart in the reference, which runs one job per process):
- * one Trotter step of several jobs, each as bqa_run_layer would run it, with their
- * phases on the batch's stream.  The extended messages and canonicalisers of all
- * jobs run before one synchronisation for the truncation rule; each BP sweep is one
+ * one Trotter step of several jobs, each as bqa_run_layer would run it.  Each job's
+ * simple update runs on the job's own stream (extended messages and canonicalisers
+ * of all jobs before the truncation rules' readbacks, then the local steps); BP runs
+ * on the batch's stream once every job's local step is done.  Each BP sweep is one
  * launch per (precision, degree, chi) shape over the classes of every job that has
  * it (k_small / k_mfma8 shapes; a job's other classes keep their own launches), with
  * each job's convergence and iteration rules applied to its own status after every

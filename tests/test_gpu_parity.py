@@ -388,13 +388,16 @@ def test_fast_kernel_phases_match_generic(name, precision):
 
 
 SMALL_CASES = {
-    # (instance, chi): degrees 2-4 on the grid, 1-3 on heavy-hex, 3 on random regular graphs
+    # (instance, chi): degrees 2-4 on the grid, 1-3 on heavy-hex, 3 and 5 on random regular graphs
     "grid4x5_chi2": (lambda: instances.grid_gaussian(4, 5, seed=31, field_sigma=0.2), 2),
     "grid4x5_chi3": (lambda: instances.grid_gaussian(4, 5, seed=32, field_sigma=0.2), 3),
     "grid4x5_chi4": (lambda: instances.grid_gaussian(4, 5, seed=33, field_sigma=0.2), 4),
     "heavyhex_chi5": (lambda: instances.heavy_hex_gaussian(2, 3, seed=34, field_sigma=0.5), 5),
     "rr24_chi6": (lambda: instances.regular_uniform(24, 3, seed=35, field_sigma=0.4), 6),
     "rr24_chi7": (lambda: instances.regular_uniform(24, 3, seed=36, field_sigma=0.4), 7),
+    # degree 5: the FP32 BP sweep runs k_small (chi <= 4); FP64 and the other phases run generic kernels
+    "rr20_d5_chi3": (lambda: instances.regular_uniform(20, 5, seed=37, field_sigma=0.4), 3),
+    "rr20_d5_chi4": (lambda: instances.regular_uniform(20, 5, seed=38, field_sigma=0.4), 4),
 }
 
 
