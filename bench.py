@@ -188,9 +188,13 @@ def accuracy(fast, ref):
             "energy_rel_err": e_rel}
 
 
-def load_traffic(precision):
-    """HBM bytes per BP-sweep launch measured with rocprofv3 counters (committed under profiles/)."""
-    path = os.path.join(ROOT, "profiles", f"bp_sweep_traffic_{TAG[precision]}.json")
+def load_traffic(precision, chi):
+    """HBM bytes per BP-sweep launch of the chi = 8 (or chi = 4) d = 3 kernel, measured with rocprofv3 counters
+    (committed under profiles/); None for other shapes."""
+    suffix = {8: "", 4: "_chi4"}.get(chi)
+    if suffix is None:
+        return None, None
+    path = os.path.join(ROOT, "profiles", f"bp_sweep_traffic_{TAG[precision]}{suffix}.json")
     if not os.path.exists(path):
         return None, None
     try:
@@ -212,7 +216,7 @@ def roofline_of(summary, ctx, chi, precision, kernel):
     flops_launch = flops_sweep / max(1, len(classes))
     gbs = (bytes_ / 1e9) / sec if sec > 0 else 0.0
     tflops = flops_launch * launches / sec / 1e12 if sec > 0 else 0.0
-    traffic, src = load_traffic(precision)
+    traffic, src = load_traffic(precision, chi)
     peak = MFMA_PEAK_TFLOPS[precision]
     return {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": src,
@@ -321,6 +325,8 @@ def main():
     ap.add_argument("--no-chi4", action="store_true")
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-hhex", action="store_true", help="skip the heavy-hex chi=16 line")
+    ap.add_argument("--hhex-precisions", default="single,double",
+                    help="precisions of the heavy-hex line (comma-separated; the error needs both)")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the serving sample (0 skips it)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="also run the plain copy kernel on BYTES bytes (HBM counter calibration)")
@@ -408,7 +414,7 @@ def main():
     if not args.no_hhex:
         hx, runs = {}, {}
         nq = None
-        for prec in ("single", "double"):
+        for prec in [p for p in args.hhex_precisions.split(",") if p]:
             ctx = config_to_context(hhex_config(HHEX_CHI, HHEX_STEPS, prec))
             nq = ctx.nodes_number
             log(f"[bench] heavy-hex {nq} qubits chi={HHEX_CHI} {TAG[prec]}: {HHEX_STEPS}-step anneal, "
@@ -423,7 +429,7 @@ def main():
             log(f"[bench] heavy-hex {TAG[prec]}: {hs['qubit_steps_per_s']:.4g} qubit*steps/s, "
                 f"{hs['ms_per_step']:.2f} ms/step, BP MFMA frac {hs['roofline']['mfma_frac']:.3f}; full anneal "
                 f"{hs['full_anneal']['seconds']:.1f} s")
-        if not args.no_accuracy:
+        if not args.no_accuracy and "single" in runs and "double" in runs:
             hx["fp32_vs_fp64_gpu"] = accuracy(runs["single"], runs["double"])
             log(f"[bench] heavy-hex fp32 vs fp64: {hx['fp32_vs_fp64_gpu']}")
         hx["workload"] = (f"heavy-hex lattice of {HHEX_CELLS}x{HHEX_CELLS} cells ({nq} qubits, degrees 2/3), "
@@ -475,6 +481,10 @@ def main():
         "serving": serving,
     }
     result.update(extra)
+    from bqa_amd.native import tuning_report
+    result["autotuner"] = [{"precision": r["precision"], "degree": r["degree"], "chi": r["chi"], "n_log2": r["n_log2"],
+                            "chosen": r["variants"][r["chosen"]]["name"],
+                            "us": {v["name"]: v["us"] for v in r["variants"]}} for r in tuning_report()]
     if copy_ms is not None:
         result["copy_calibration"] = {"bytes": args.copy_calibrate, "ms_per_copy": copy_ms,
                                       "GB_per_s": 2 * args.copy_calibrate / 1e6 / copy_ms}

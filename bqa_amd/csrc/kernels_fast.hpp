@@ -97,11 +97,13 @@ inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA
 //   chi = 16, d = 3: k_mfma16_bp<3, W>, W = 8 or 4 waves per node (one node per CU, 128 KB LDS);
 //   chi = 16, d = 2: k_mfma16_bp<2, 1>, one or two resident rounds;
 //   chi = 8, d = 2, 3: k_mfma8 (one wave per node) under the three grid policies;
-//   other served shapes: k_small under the three grid policies.
+//   other served shapes: k_small under the three grid policies, and for degree 2, 3 with chi <= 4
+//   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<float>(int d, int chi) {
   if (chi == 16 && (d == 2 || d == 3)) return 2;
   if (chi == 8 && (d == 2 || d == 3)) return kGridPolicies;
+  if (npw_shape(d, chi)) return kGridPolicies + 2;
   return small_bp_shape<float>(d, chi) ? kGridPolicies : 1;
 }
 
@@ -111,9 +113,11 @@ inline const char* bp_variant_name<float>(int d, int chi, int v) {
   if (chi == 16 && d == 2) return v == 0 ? "k_mfma16_bp<2,1>, resident" : "k_mfma16_bp<2,1>, 2x resident";
   static const char* m8[kGridPolicies] = {"k_mfma8, resident SIMD-rounded", "k_mfma8, resident",
                                           "k_mfma8, 2x resident"};
-  static const char* sm[kGridPolicies] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident"};
+  static const char* sm[kGridPolicies + 2] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident",
+                                              "k_small_npw<2> (2 nodes per wave), resident",
+                                              "k_small_npw<4> (4 nodes per wave), resident"};
   if (chi == 8 && (d == 2 || d == 3)) return m8[v % kGridPolicies];
-  return small_bp_shape<float>(d, chi) ? sm[v % kGridPolicies] : "generic";
+  return small_bp_shape<float>(d, chi) ? sm[v % (kGridPolicies + 2)] : "generic";
 }
 
 // FP32, chi = 8, degree 2 or 3: k_mfma8, one wave per node (two and four waves per node for
@@ -139,6 +143,9 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
     }
     return true;
   }
+  if (v >= kGridPolicies && npw_shape(cv.d, chi))
+    return v == kGridPolicies ? launch_small_npw<2, float>(cv, chi, T, mA, mB, st, s)
+                              : launch_small_npw<4, float>(cv, chi, T, mA, mB, st, s);
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s, v);
   if (cv.d == 2)
@@ -152,11 +159,13 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
 //   chi = 8, d = 3: k_mfma8_bp64_3p<4> (six contractions per node over four waves) under the three
 //     grid policies, and k_mfma8_bp64_3p<2> (two waves per node);
 //   chi = 8, d = 2: k_mfma8_bp64<2, 1> (one wave per node) under the three grid policies;
-//   other served shapes: k_small under the three grid policies.
+//   other served shapes: k_small under the three grid policies, and for degree 2, 3 with chi <= 4
+//   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<double>(int d, int chi) {
   if (chi == 8 && d == 3) return kGridPolicies + 1;
   if (chi == 8 && d == 2) return kGridPolicies;
+  if (npw_shape(d, chi)) return kGridPolicies + 2;
   return small_shape(d, chi) ? kGridPolicies : 1;
 }
 
@@ -167,15 +176,20 @@ inline const char* bp_variant_name<double>(int d, int chi, int v) {
                                               "k_mfma8_bp64_3p<2>, resident SIMD-rounded"};
   static const char* b2[kGridPolicies] = {"k_mfma8_bp64<2,1>, resident SIMD-rounded", "k_mfma8_bp64<2,1>, resident",
                                           "k_mfma8_bp64<2,1>, 2x resident"};
-  static const char* sm[kGridPolicies] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident"};
+  static const char* sm[kGridPolicies + 2] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident",
+                                              "k_small_npw<2> (2 nodes per wave), resident",
+                                              "k_small_npw<4> (4 nodes per wave), resident"};
   if (chi == 8 && d == 3) return b3[v % (kGridPolicies + 1)];
   if (chi == 8 && d == 2) return b2[v % kGridPolicies];
-  return small_shape(d, chi) ? sm[v % kGridPolicies] : "generic";
+  return small_shape(d, chi) ? sm[v % (kGridPolicies + 2)] : "generic";
 }
 
 template <>
 inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
                                    BpStatus* st, hipStream_t s, int v) {
+  if (v >= kGridPolicies && npw_shape(cv.d, chi))
+    return v == kGridPolicies ? launch_small_npw<2, double>(cv, chi, T, mA, mB, st, s)
+                              : launch_small_npw<4, double>(cv, chi, T, mA, mB, st, s);
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeBP, double, double>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s, v);
   if (cv.d == 3) {

@@ -409,6 +409,156 @@ __global__ __launch_bounds__(64) void k_small_multi(const BpEntry* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// BP sweep with NPW nodes per wave (chi <= 4, degree 2 and 3): each node gets
+// GL = 64 / NPW lanes, its own LDS tensor and work buffers, and the same
+// schedule as k_small; the wave's phases (and their barriers) serve NPW nodes
+// at once, and at GL >= Q a Gram entry is summed by one lane over all fibres
+// (no cross-lane reduction).  Same arithmetic per node as k_small, in the same
+// order within each lane except the Gram's fibre split.
+template <int D, int CHI, int NPW, typename C>
+struct NpwSmem {
+  cplx<C> T[NPW][Sm<D, CHI>::S];
+  cplx<C> X[NPW][Sm<D, CHI>::S];
+  cplx<C> Y[NPW][Sm<D, CHI>::S];
+  cplx<C> m[NPW][D][CHI * CHI];
+  cplx<C> red[NPW][CHI * CHI];
+};
+
+// Y = X x_J m for this lane group's node (gl: lane in the group of GL lanes).
+template <int D, int CHI, int J, int GL, typename C>
+__device__ __forceinline__ void smp_g(const cplx<C>* X, cplx<C>* Y, const cplx<C>* m, int gl) {
+  constexpr int S = Sm<D, CHI>::S, st = cpow(CHI, D - 1 - J);
+#pragma unroll
+  for (int q = 0; q < (S + GL - 1) / GL; ++q) {
+    const int idx = gl + GL * q;
+    if (S % GL == 0 || idx < S) {
+      const int a = (idx / st) % CHI;
+      const int base = idx - a * st;
+      cplx<C> acc = mk<C>(0, 0);
+#pragma unroll
+      for (int b = 0; b < CHI; ++b) cfma(acc, m[a * CHI + b], X[base + b * st]);
+      Y[idx] = acc;
+    }
+  }
+}
+
+// Gram along bond K of this group's node; lanes gl < Q end with the full sum.
+template <int D, int CHI, int K, int GL, typename C>
+__device__ __forceinline__ cplx<C> sgram_g(const cplx<C>* T, const cplx<C>* Y, int gl) {
+  using M = Sm<D, CHI>;
+  // fibre groups per entry: GL / Q for a power-of-two Q (then a power of two), else one
+  constexpr int Q = M::Q, NG = (Q & (Q - 1)) == 0 ? GL / Q : 1, sk = cpow(CHI, D - 1 - K);
+  static_assert(NG >= 1 && (NG & (NG - 1)) == 0, "a power-of-two number of fibre groups per Gram entry");
+  const int o = gl % Q, grp = gl / Q;
+  const int c = o / CHI, e = o % CHI;
+  cplx<C> g = mk<C>(0, 0);
+  if (grp < NG) {
+#pragma unroll 8
+    for (int r0 = 0; r0 < M::RT; r0 += NG) {
+      const int r = r0 + grp;
+      if (M::RT % NG == 0 || r < M::RT) {
+        const int base = (r / sk) * CHI * sk + r % sk;
+        cfma_cj(g, T[base + c * sk], Y[base + e * sk]);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = Q; off < Q * NG; off <<= 1) {  // NG > 1 only for a power-of-two Q: stays inside the group
+    g.x += __shfl_xor(g.x, off, 64);
+    g.y += __shfl_xor(g.y, off, 64);
+  }
+  return g;
+}
+
+template <int D, int CHI, int K, int NPW, typename R, typename C>
+__device__ __forceinline__ void sout_g(NpwSmem<D, CHI, NPW, C>& sm, const cplx<C>* Y, const ClassView& cv, int i,
+                                       bool valid, int gi, int gl, const cplx<R> (&old)[D], cplx<R>* mout,
+                                       double& dnum, double& dden) {
+  constexpr int GL = 64 / NPW, Q = CHI * CHI;
+  const cplx<C> g = sgram_g<D, CHI, K, GL, C>(sm.T[gi], Y, gl);
+  if (gl < Q) sm.red[gi][gl] = g;
+  small_sync();
+  if (valid && gl < Q) {
+    const int c = gl / CHI, e = gl % CHI;
+    double tr = 0.0;
+#pragma unroll
+    for (int k = 0; k < CHI; ++k) tr += (double)sm.red[gi][k * CHI + k].x;
+    const double inv = 1.0 / tr;
+    const cplx<double> a = cast<double>(sm.red[gi][gl]), b = cast<double>(sm.red[gi][e * CHI + c]);
+    const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
+    const cplx<double> vd = cast<double>(v), od = cast<double>(old[K]);
+    dnum = fmax(dnum, abs2(vd - od));
+    dden = fmax(dden, abs2(vd + od));
+    mout[(long)cv.out_pos[K * cv.n + i] * Q + gl] = v;
+  }
+  small_sync();
+}
+
+template <int D, int CHI, int NPW, typename R>
+__global__ __launch_bounds__(64) void k_small_npw(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
+                                                  cplx<R>* msgB, BpStatus* st) {
+  static_assert((D == 2 || D == 3) && CHI >= 2 && CHI <= 4, "shape");
+  using C = R;
+  using M = Sm<D, CHI>;
+  constexpr int GL = 64 / NPW;
+  __shared__ __align__(16) NpwSmem<D, CHI, NPW, C> sm;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const cplx<R>* min = cur ? msgB : msgA;
+  cplx<R>* mout = cur ? msgA : msgB;
+  const int n = cv.n;
+  const int l = threadIdx.x & 63, gi = l / GL, gl = l % GL;
+  double dnum = 0.0, dden = 0.0;
+  for (long i0 = (long)blockIdx.x * NPW; i0 < n; i0 += (long)gridDim.x * NPW) {
+    const int i = (int)i0 + gi;
+    const bool valid = i < n;
+    const int ic = valid ? i : n - 1;  // a group past the end repeats the last node and stores nothing
+    const cplx<R>* Tg = T + (size_t)ic * M::S;
+#pragma unroll
+    for (int q = 0; q < (M::S + GL - 1) / GL; ++q) {
+      const int x = gl + GL * q;
+      if (M::S % GL == 0 || x < M::S) sm.T[gi][x] = cast<C>(Tg[x]);
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const cplx<R>* mj = min + (size_t)cv.in_pos[j * n + ic] * M::Q;
+      if (gl < M::Q) sm.m[gi][j][gl] = cast<C>(mj[gl]);
+    }
+    cplx<R> old[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) old[k] = gl < M::Q ? min[(size_t)cv.out_pos[k * n + ic] * M::Q + gl] : mk<R>(0, 0);
+    small_sync();
+    if constexpr (D == 2) {
+      smp_g<D, CHI, 1, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][1], gl);  // Y0 = T x1 m1
+      small_sync();
+      sout_g<D, CHI, 0, NPW, R, C>(sm, sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      smp_g<D, CHI, 0, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][0], gl);  // Y1 = T x0 m0
+      small_sync();
+      sout_g<D, CHI, 1, NPW, R, C>(sm, sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+    } else {
+      smp_g<D, CHI, 2, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][2], gl);  // W = T x2 m2
+      small_sync();
+      smp_g<D, CHI, 1, GL, C>(sm.X[gi], sm.Y[gi], sm.m[gi][1], gl);  // Y0 = W x1 m1
+      small_sync();
+      sout_g<D, CHI, 0, NPW, R, C>(sm, sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      smp_g<D, CHI, 0, GL, C>(sm.X[gi], sm.Y[gi], sm.m[gi][0], gl);  // Y1 = W x0 m0
+      small_sync();
+      sout_g<D, CHI, 1, NPW, R, C>(sm, sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      smp_g<D, CHI, 0, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][0], gl);  // W = T x0 m0
+      small_sync();
+      smp_g<D, CHI, 1, GL, C>(sm.X[gi], sm.Y[gi], sm.m[gi][1], gl);  // Y2 = W x1 m1
+      small_sync();
+      sout_g<D, CHI, 2, NPW, R, C>(sm, sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+    dden = fmax(dden, __shfl_xor(dden, o, 64));
+  }
+  if (l == 0) bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
+}
+
 // Shapes served: degree 2 and 3 with chi 2..8, degree 4 with chi 2..4.
 inline bool small_shape(int d, int chi) {
   return chi >= 2 && ((d == 2 || d == 3) ? chi <= 8 : (d == 4 && chi <= 4));
@@ -478,6 +628,31 @@ inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, c
     case 4: return launch_small_d<4, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
     default: return false;
   }
+}
+
+// Shapes of k_small_npw: degree 2 and 3 with chi 2..4.
+inline bool npw_shape(int d, int chi) { return (d == 2 || d == 3) && chi >= 2 && chi <= 4; }
+
+template <int D, int CHI, int NPW, typename R>
+inline void launch_small_npw_dc(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
+                                hipStream_t s) {
+  const int groups = (cv.n + NPW - 1) / NPW;
+  hipLaunchKernelGGL((k_small_npw<D, CHI, NPW, R>), dim3(policy_grid<k_small_npw<D, CHI, NPW, R>, 64>(groups, 0)),
+                     dim3(64), 0, s, cv, T, mA, mB, st);
+}
+
+template <int NPW, typename R>
+inline bool launch_small_npw(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st,
+                             hipStream_t s) {
+  if (cv.n == 0 || !npw_shape(cv.d, chi)) return false;
+#define BQA_NPW(DD, CC)                                                        \
+  if (cv.d == DD && chi == CC) {                                               \
+    launch_small_npw_dc<DD, CC, NPW, R>(cv, T, mA, mB, st, s);                 \
+    return true;                                                               \
+  }
+  BQA_NPW(2, 2) BQA_NPW(2, 3) BQA_NPW(2, 4) BQA_NPW(3, 2) BQA_NPW(3, 3) BQA_NPW(3, 4)
+#undef BQA_NPW
+  return false;
 }
 
 // Fused absorb + Z/X gates + symmetric gauge + message re-initialisation at

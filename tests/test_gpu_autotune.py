@@ -20,8 +20,15 @@ pytestmark = pytest.mark.gpu
 
 # name: (instance, chi, precision, variants, tolerance of the wave-split variants relative to max |m|)
 CASES = {
-    "rr12000_chi4_fp32": (lambda: instances.regular_uniform(12000, 3, seed=41, field_sigma=0.3), 4, "single", 3, 0.0),
-    "rr12000_chi4_fp64": (lambda: instances.regular_uniform(12000, 3, seed=41, field_sigma=0.3), 4, "double", 3, 0.0),
+    # chi <= 4, degree 2 and 3: grid policies, then k_small_npw with 2 and 4 nodes per wave (another
+    # fibre split of the Grams)
+    "rr12000_chi4_fp32": (lambda: instances.regular_uniform(12000, 3, seed=41, field_sigma=0.3), 4, "single", 5,
+                          2e-6),
+    "rr12000_chi4_fp64": (lambda: instances.regular_uniform(12000, 3, seed=41, field_sigma=0.3), 4, "double", 5,
+                          1e-13),
+    "rr9000_chi3_fp32": (lambda: instances.regular_uniform(9000, 3, seed=45, field_sigma=0.3), 3, "single", 5, 2e-6),
+    "grid90x90_chi2_fp64": (lambda: instances.grid_gaussian(90, 90, seed=46, field_sigma=0.1), 2, "double", 5,
+                            1e-13),
     "rr6000_chi8_fp32": (lambda: instances.regular_uniform(6000, 3, seed=42, field_sigma=0.3), 8, "single", 3, 0.0),
     "rr6000_chi8_fp64": (lambda: instances.regular_uniform(6000, 3, seed=42, field_sigma=0.3), 8, "double", 4, 1e-13),
     "heavyhex_chi8_fp64": (lambda: instances.heavy_hex_gaussian(24, 24, seed=43, field_sigma=0.5), 8, "double", 4,
