@@ -1,0 +1,266 @@
+// BP sweep for degree-4 nodes at chi = 8 (the 100x100 grid's interior, the d = 4
+// serving jobs).  A node tensor is 2 x 8^4 complex elements (128 KB in FP64), so
+// the kernel works on one physical half T_p (4096 elements) at a time, in two
+// LDS buffers A and B, with mode products done in place (each thread owns whole
+// fibres along the product's bond, reads them, then writes them back).
+//
+// The reference computes out_k = Gram over every bond but k of conj(T) against
+// T contracted with the three other incoming messages (backends.py:381-404).
+// Every message is Hermitian, so a message can act on the bra side instead:
+// sum_a conj(T[..a..]) m[a][b] = conj((T x m)[..b..]).  With that, per half:
+//   B = T x2 m2 x3 m3;  out_0 += conj(T x1 m1) . B;  out_1 += conj(T x0 m0) . B;
+//   B = T x0 m0 x1 m1;  out_2 += conj(T x3 m3) . B;  out_3 += conj(T x2 m2) . B;
+// eight mode products and four Grams per half (the reference's schedule: twelve
+// and four).  T_p is read again from global memory (L2) for each product chain.
+#pragma once
+
+#include "common.hpp"
+#include "kernels_node.hpp"
+#include "launch_grid.hpp"
+
+namespace bqa {
+
+// LDS position of half-tensor element idx: the low three bits (the last bond's index) are XORed with the
+// next three, a permutation inside each aligned row of eight, so the fibres along the last bond, which one
+// thread reads whole, fall on different banks across a wave; every other access pattern here reads whole
+// aligned rows and stays conflict-free.
+__device__ __forceinline__ int d4_sw(int idx) { return idx ^ ((idx >> 3) & 7); }
+
+template <typename C>
+struct D4Lds {
+  static constexpr int H = 4096;                  // elements of one physical half
+  static constexpr int BYTES = 2 * H * (int)sizeof(cplx<C>) + 4 * 64 * (int)sizeof(cplx<C>);
+};
+
+// In-place mode product along bond J (stride 8^(3-J) within a half): X[.., a, ..] <- sum_b m[a][b] X[.., b, ..].
+template <int J, int NT, typename C>
+__device__ __forceinline__ void d4_mp(cplx<C>* X, const cplx<C>* __restrict__ m) {
+  constexpr int st = J == 0 ? 512 : J == 1 ? 64 : J == 2 ? 8 : 1;
+#pragma unroll
+  for (int q = 0; q < 512 / NT; ++q) {
+    const int f = threadIdx.x + NT * q;   // fibre
+    const int base = (f / st) * 8 * st + f % st;
+    cplx<C> x[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) x[b] = X[d4_sw(base + b * st)];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      cplx<C> acc = mk<C>(0, 0);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) cfma(acc, m[a * 8 + b], x[b]);
+      X[d4_sw(base + a * st)] = acc;
+    }
+  }
+}
+
+// Gram along bond K: g += sum over this thread's fibre group r of conj(A[r, c]) B[r, e], entry (c, e) =
+// (threadIdx % 64) / 8, % 8; the NT / 64 groups of 64 threads take every (NT / 64)-th fibre.
+template <int K, int NT, typename C>
+__device__ __forceinline__ void d4_gram(const cplx<C>* A, const cplx<C>* B, cplx<double>& g) {
+  constexpr int st = K == 0 ? 512 : K == 1 ? 64 : K == 2 ? 8 : 1;
+  const int o = threadIdx.x & 63, grp = threadIdx.x >> 6, c = o >> 3, e = o & 7;
+  cplx<C> acc = mk<C>(0, 0);
+#pragma unroll 8
+  for (int r = grp; r < 512; r += NT / 64) {
+    const int base = (r / st) * 8 * st + r % st;
+    cfma_cj(acc, A[d4_sw(base + c * st)], B[d4_sw(base + e * st)]);
+  }
+  g = g + cast<double>(acc);
+}
+
+__device__ __forceinline__ void d4_sync() { __syncthreads(); }
+
+// One physical half T_p of a node: the schedule above, adding the half's four Grams into g[k].
+// M holds the four incoming messages; on return A and B are still being read by other threads'
+// last Gram (the caller synchronises before reusing them).
+template <int NT, typename R, typename C>
+__device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M, const cplx<R>* __restrict__ Tp,
+                                        cplx<double> (&g)[4]) {
+  constexpr int H = D4Lds<C>::H;
+  const int t = threadIdx.x;
+  auto load = [&](cplx<C>* X) {
+#pragma unroll
+    for (int q = 0; q < H / NT; ++q) X[d4_sw(t + NT * q)] = cast<C>(Tp[t + NT * q]);
+  };
+  load(B);
+  d4_sync();
+  d4_mp<2, NT, C>(B, M + 2 * 64);
+  d4_sync();
+  d4_mp<3, NT, C>(B, M + 3 * 64);   // B = T x2 m2 x3 m3
+  load(A);
+  d4_sync();
+  d4_mp<1, NT, C>(A, M + 1 * 64);   // A = T x1 m1
+  d4_sync();
+  d4_gram<0, NT, C>(A, B, g[0]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4_mp<0, NT, C>(A, M + 0 * 64);   // A = T x0 m0
+  d4_sync();
+  d4_gram<1, NT, C>(A, B, g[1]);
+  d4_sync();
+  load(B);
+  d4_sync();
+  d4_mp<0, NT, C>(B, M + 0 * 64);
+  d4_sync();
+  d4_mp<1, NT, C>(B, M + 1 * 64);   // B = T x0 m0 x1 m1
+  load(A);
+  d4_sync();
+  d4_mp<3, NT, C>(A, M + 3 * 64);   // A = T x3 m3
+  d4_sync();
+  d4_gram<2, NT, C>(A, B, g[2]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4_mp<2, NT, C>(A, M + 2 * 64);   // A = T x2 m2
+  d4_sync();
+  d4_gram<3, NT, C>(A, B, g[3]);
+}
+
+// NT threads per node (256 or 512: one or two waves per SIMD, the node filling the CU's LDS in FP64).
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_bp_d4c8(ClassView cv, const cplx<R>* __restrict__ T,
+                                                cplx<R>* msgA, cplx<R>* msgB, BpStatus* st) {
+  static_assert(NT == 256 || NT == 512, "four or eight waves per node");
+  using C = R;
+  constexpr int H = D4Lds<C>::H;
+  extern __shared__ __align__(16) unsigned char d4_lds[];
+  cplx<C>* A = reinterpret_cast<cplx<C>*>(d4_lds);
+  cplx<C>* B = A + H;
+  cplx<C>* M = B + H;  // the four incoming messages, 64 entries each
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const cplx<R>* min = cur ? msgB : msgA;
+  cplx<R>* mout = cur ? msgA : msgB;
+  const int n = cv.n;
+  const int t = threadIdx.x, o = t & 63;
+  double dnum = 0.0, dden = 0.0;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    d4_sync();  // the previous node's last readers of A, B and M are done
+    if (t < 256) {
+      const int j = t >> 6;
+      M[j * 64 + o] = cast<C>(min[(size_t)cv.in_pos[j * n + i] * 64 + o]);
+    }
+    cplx<double> g[4] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      if (p) d4_sync();  // the last Gram of the first half has read B
+      d4_half<NT, R, C>(A, B, M, T + (size_t)i * 2 * H + (size_t)p * H, g);
+    }
+    // partial Grams of the four fibre groups into the dead A buffer (FP64), then one wave finishes
+    d4_sync();
+    constexpr int NG = NT / 64;  // fibre groups
+    cplx<double>* red = reinterpret_cast<cplx<double>*>(A);  // [k][grp][64]: at most 32 KB <= the A buffer
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[(k * NG + (t >> 6)) * 64 + o] = g[k];
+    d4_sync();
+    if (t < 64) {
+      const int c = o >> 3, e = o & 7;
+#pragma unroll 1
+      for (int k = 0; k < 4; ++k) {
+        auto entry = [&](int x) {
+          cplx<double> s = mk(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < NG; ++q) s = s + red[(k * NG + q) * 64 + x];
+          return s;
+        };
+        double tr = 0.0;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) tr += entry(d * 9).x;
+        const double inv = 1.0 / tr;
+        const cplx<double> a = entry(o), b = entry(e * 8 + c);
+        const cplx<R> v = cast<R>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
+        const size_t pos = (size_t)cv.out_pos[k * n + i];
+        const cplx<double> vd = cast<double>(v), od = cast<double>(min[pos * 64 + o]);
+        dnum = fmax(dnum, abs2(vd - od));
+        dden = fmax(dden, abs2(vd + od));
+        mout[pos * 64 + o] = v;
+      }
+    }
+  }
+  if (t < 64) {
+    for (int s = 32; s > 0; s >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, s, 64));
+      dden = fmax(dden, __shfl_xor(dden, s, 64));
+    }
+    if (t == 0) bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
+  }
+}
+
+template <typename R, int NT>
+inline void launch_bp_d4c8(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>* mB, BpStatus* st, hipStream_t s) {
+  constexpr int LDS = D4Lds<R>::BYTES;
+  const int grid = node_grid_lds<k_bp_d4c8<R, NT>, NT, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_bp_d4c8<R, NT>), dim3(grid), dim3(NT), LDS, s, cv, T, mA, mB, st);
+}
+
+
+// Extended messages of degree-4 nodes at chi = 8 (state.py:127-139, backends.py:393-404,519-526), FP64
+// arithmetic on R storage: the same products per physical half, the Grams kept per half (p), and each
+// 16 x 16 extended message assembled from the two halves and the bond's ZZ-split gates g0, g1:
+// ext[(s,c),(s',e)] = conj(g_s) g_s' (G_0 +- G_1)[c][e] / (trace * (|g0|^2 + |g1|^2)), with + for
+// s + s' even.  One node per block, NT threads.
+template <typename R, int NT>
+__global__ __launch_bounds__(NT) void k_ext_d4c8(ClassView cv, const cplx<R>* __restrict__ T,
+                                                 const cplx<R>* __restrict__ msgs, cplx<double>* ext_out) {
+  static_assert(NT == 256 || NT == 512, "four or eight waves per node");
+  using C = double;
+  constexpr int H = D4Lds<C>::H, NG = NT / 64;
+  extern __shared__ __align__(16) unsigned char d4_lds[];
+  cplx<C>* A = reinterpret_cast<cplx<C>*>(d4_lds);
+  cplx<C>* B = A + H;
+  cplx<C>* M = B + H;
+  const int n = cv.n;
+  const int t = threadIdx.x, o = t & 63;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    d4_sync();  // the previous node's last readers of A, B and M are done
+    if (t < 256) {
+      const int j = t >> 6;
+      M[j * 64 + o] = cast<C>(msgs[(size_t)cv.in_pos[j * n + i] * 64 + o]);
+    }
+    cplx<double> g[2][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) g[p][k] = mk(0.0, 0.0);
+    d4_half<NT, R, C>(A, B, M, T + (size_t)i * 2 * H, g[0]);
+    d4_sync();
+    d4_half<NT, R, C>(A, B, M, T + (size_t)i * 2 * H + H, g[1]);
+    d4_sync();
+    cplx<double>* red = A;  // [k][p][grp][64]: at most 64 KB, the A buffer
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) red[((k * 2 + p) * NG + (t >> 6)) * 64 + o] = g[p][k];
+    d4_sync();
+    for (int x = t; x < 4 * 256; x += NT) {
+      const int k = x >> 8, e16 = x & 255, row = e16 >> 4, col = e16 & 15;
+      const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
+      auto sum = [&](int p, int entry) {
+        cplx<double> v = mk(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < NG; ++q) v = v + red[((k * 2 + p) * NG + q) * 64 + entry];
+        return v;
+      };
+      cplx<double> tr = mk(0.0, 0.0);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) tr = tr + sum(0, d * 9) + sum(1, d * 9);
+      const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
+      const cplx<double> g0 = zg[0], g1 = zg[1];
+      const cplx<double> inv = cinv(tr * mk(abs2(g0) + abs2(g1), 0.0));
+      const cplx<double> a0 = sum(0, c * 8 + e), a1 = sum(1, c * 8 + e);
+      const cplx<double> gs = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
+      const cplx<double> ga = s ? g1 : g0, gb = s2 ? g1 : g0;
+      ext_out[(size_t)cv.out_pos[k * n + i] * 256 + e16] = conj(ga) * gb * gs * inv;
+    }
+  }
+}
+
+template <typename R, int NT>
+inline void launch_ext_d4c8(ClassView cv, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext, hipStream_t s) {
+  constexpr int LDS = D4Lds<double>::BYTES;
+  const int grid = node_grid_lds<k_ext_d4c8<R, NT>, NT, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_ext_d4c8<R, NT>), dim3(grid), dim3(NT), LDS, s, cv, T, msgs, ext);
+}
+
+}  // namespace bqa

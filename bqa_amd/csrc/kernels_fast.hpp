@@ -9,6 +9,7 @@
 #include "kernels_mfma.hpp"
 #include "kernels_mfma16.hpp"
 #include "kernels_small.hpp"
+#include "kernels_d4.hpp"
 
 namespace bqa {
 
@@ -30,6 +31,7 @@ int bp_variant_count(int d, int chi);
 template <typename R>
 inline bool bp_fast_shape(int d, int chi) {
   if ((chi == 8 || (sizeof(R) == sizeof(float) && chi == 16)) && (d == 2 || d == 3)) return true;
+  if (chi == 8 && d == 4) return true;
   return small_bp_shape<R>(d, chi);
 }
 
@@ -60,6 +62,10 @@ template <typename R>
 bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext,
                      hipStream_t s) {
   if (cv.n == 0 || cv.zz == nullptr) return false;
+  if (chi == 8 && cv.d == 4) {
+    launch_ext_d4c8<R, 512>(cv, T, msgs, ext, s);
+    return true;
+  }
   if (chi == 8 && (cv.d == 2 || cv.d == 3)) {
     if constexpr (sizeof(R) == sizeof(float)) {
       // FP32 job: FP32 MFMA mode products, FP64 MFMA Gram
@@ -97,11 +103,13 @@ inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA
 //   chi = 16, d = 3: k_mfma16_bp<3, W>, W = 8 or 4 waves per node (one node per CU, 128 KB LDS);
 //   chi = 16, d = 2: k_mfma16_bp<2, 1>, one or two resident rounds;
 //   chi = 8, d = 2, 3: k_mfma8 (one wave per node) under the three grid policies;
+//   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node;
 //   other served shapes: k_small under the three grid policies, and for degree 2, 3 with chi <= 4
 //   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<float>(int d, int chi) {
   if (chi == 16 && (d == 2 || d == 3)) return 2;
+  if (chi == 8 && d == 4) return 2;
   if (chi == 8 && (d == 2 || d == 3)) return kGridPolicies;
   if (npw_shape(d, chi)) return kGridPolicies + 2;
   return small_bp_shape<float>(d, chi) ? kGridPolicies : 1;
@@ -111,6 +119,7 @@ template <>
 inline const char* bp_variant_name<float>(int d, int chi, int v) {
   if (chi == 16 && d == 3) return v == 0 ? "k_mfma16_bp<3,8>" : "k_mfma16_bp<3,4>";
   if (chi == 16 && d == 2) return v == 0 ? "k_mfma16_bp<2,1>, resident" : "k_mfma16_bp<2,1>, 2x resident";
+  if (chi == 8 && d == 4) return v == 0 ? "k_bp_d4c8<256 threads>" : "k_bp_d4c8<512 threads>";
   static const char* m8[kGridPolicies] = {"k_mfma8, resident SIMD-rounded", "k_mfma8, resident",
                                           "k_mfma8, 2x resident"};
   static const char* sm[kGridPolicies + 2] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident",
@@ -146,6 +155,13 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
   if (v >= kGridPolicies && npw_shape(cv.d, chi))
     return v == kGridPolicies ? launch_small_npw<2, float>(cv, chi, T, mA, mB, st, s)
                               : launch_small_npw<4, float>(cv, chi, T, mA, mB, st, s);
+  if (chi == 8 && cv.d == 4 && cv.n > 0) {
+    if (v == 1)
+      launch_bp_d4c8<float, 512>(cv, T, mA, mB, st, s);
+    else
+      launch_bp_d4c8<float, 256>(cv, T, mA, mB, st, s);
+    return true;
+  }
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeBP, float, float>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s, v);
   if (cv.d == 2)
@@ -159,10 +175,12 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
 //   chi = 8, d = 3: k_mfma8_bp64_3p<4> (six contractions per node over four waves) under the three
 //     grid policies, and k_mfma8_bp64_3p<2> (two waves per node);
 //   chi = 8, d = 2: k_mfma8_bp64<2, 1> (one wave per node) under the three grid policies;
+//   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node;
 //   other served shapes: k_small under the three grid policies, and for degree 2, 3 with chi <= 4
 //   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<double>(int d, int chi) {
+  if (chi == 8 && d == 4) return 2;
   if (chi == 8 && d == 3) return kGridPolicies + 1;
   if (chi == 8 && d == 2) return kGridPolicies;
   if (npw_shape(d, chi)) return kGridPolicies + 2;
@@ -179,6 +197,7 @@ inline const char* bp_variant_name<double>(int d, int chi, int v) {
   static const char* sm[kGridPolicies + 2] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident",
                                               "k_small_npw<2> (2 nodes per wave), resident",
                                               "k_small_npw<4> (4 nodes per wave), resident"};
+  if (chi == 8 && d == 4) return v == 0 ? "k_bp_d4c8<256 threads>" : "k_bp_d4c8<512 threads>";
   if (chi == 8 && d == 3) return b3[v % (kGridPolicies + 1)];
   if (chi == 8 && d == 2) return b2[v % kGridPolicies];
   return small_shape(d, chi) ? sm[v % (kGridPolicies + 2)] : "generic";
@@ -190,6 +209,13 @@ inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T,
   if (v >= kGridPolicies && npw_shape(cv.d, chi))
     return v == kGridPolicies ? launch_small_npw<2, double>(cv, chi, T, mA, mB, st, s)
                               : launch_small_npw<4, double>(cv, chi, T, mA, mB, st, s);
+  if (chi == 8 && cv.d == 4 && cv.n > 0) {
+    if (v == 1)
+      launch_bp_d4c8<double, 512>(cv, T, mA, mB, st, s);
+    else
+      launch_bp_d4c8<double, 256>(cv, T, mA, mB, st, s);
+    return true;
+  }
   if (chi != 8 || cv.n == 0 || (cv.d != 2 && cv.d != 3))
     return launch_small<kModeBP, double, double>(cv, chi, T, mA, mB, st, nullptr, nullptr, nullptr, s, v);
   if (cv.d == 3) {
