@@ -14,8 +14,11 @@ Jobs are admitted in arrival order once they have arrived, a slot is free
   its own ``bqa_job`` handle and HIP stream, so the sweeps of concurrent jobs
   overlap on the GPU.
 
-Either way a job's results are those it gives alone (the kernels are
-deterministic and jobs share no device state).
+Either way a job's results are those it gives alone with the same BP launch
+variants (the kernels are deterministic and jobs share no device state).  The
+shared launches compute what variant 0 computes; with the autotuner on
+(``BQA_AUTOTUNE``, the default), a job alone may take a variant that orders a
+node's sums differently, and then agrees with its served run to rounding.
 
 ``serve`` returns each job's results with its arrival, start and finish
 times, and the aggregate throughput (jobs/h, qubit-steps/s) with p50/p99
