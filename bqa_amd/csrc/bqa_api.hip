@@ -816,8 +816,10 @@ __global__ void k_widen(const cplx<float>* __restrict__ src, cplx<double>* dst, 
     dst[x] = cast<double>(src[x]);
 }
 
-// Extended messages of one class in FP64 arithmetic: MFMA for chi = 8 and degree 2-3,
-// otherwise the generic kernel (FP32 inputs are widened into a scratch buffer first).
+// Extended messages of one class in FP64 arithmetic: MFMA for chi = 8 at degrees 2-3 and for
+// FP32 jobs at chi = 16, degrees 2-3 (FP32 mode products, FP64 Grams), the degree-4 chi = 8
+// kernel, the compile-time-shaped kernels below chi = 8, otherwise the generic kernel (FP32
+// inputs are widened into a scratch buffer first).
 template <typename R>
 void ext_class(bqa_job* j, ClassDev& c, double ztime, const cplx<double>* msgs64) {
   const int chi = j->chi;
@@ -870,7 +872,8 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps, bool sync = true, 
   bool need_generic = false;
   for (auto& c : j->cls)
     need_generic |= c.d != 0 && c.n > 0 &&
-                    (j->generic(kGenExt) || !((chi == 8 && c.d >= 2 && c.d <= 4) || small_shape(c.d, chi)));
+                    (j->generic(kGenExt) || !((chi == 8 && c.d >= 2 && c.d <= 4) || small_shape(c.d, chi) ||
+                                              (sizeof(R) == sizeof(float) && chi == 16 && (c.d == 2 || c.d == 3))));
   if (sizeof(R) == sizeof(float) && need_generic) {
     const long cnt = (long)j->M * chi * chi;
     j->conv_msgs.ensure((size_t)cnt * sizeof(cplx<double>), j->dev_bytes, j->stream);

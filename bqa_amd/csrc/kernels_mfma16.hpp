@@ -308,4 +308,174 @@ __global__ __launch_bounds__(64 * W) void k_mfma16_bp(ClassView cv, const cplx<f
   }
 }
 
+// ---------------------------------------------------------------------------
+// Extended messages of an FP32 job at chi = 16, degree 2 or 3 (state.py:127-139,
+// backends.py:391-408): the mode products of k_mfma16_bp in FP32 MFMA, then for
+// every bond K the two physical-index-resolved Grams
+//   G_p[c][e] = sum over fibres with physical index p of conj(T[f][c]) Y[f][e]
+// on v_mfma_f64_16x16x4f64 from the FP32 operands (the products are exact in FP64
+// and the sums keep FP64 rounding: the canonicaliser's S^{-1/2} amplifies Gram
+// rounding on small Schmidt values), and the 32 x 32 FP64 message of the bond's
+// ZZ-split gates (g_0, g_1):
+//   ext[(s, c)][(s', e)] = conj(g_s) g_s' (G_0 + (-1)^(s + s') G_1)[c][e] / (tr (|g_0|^2 + |g_1|^2)),
+// tr the complex trace of G_0 + G_1 (the ket side carries the conditional Z gate
+// sqrt(cos) I, sqrt(-i sin) Z of backends.py:592-600, the bra side its conjugate).
+//
+// The fibres of a bond are ordered with the physical index most significant, so the
+// first half of the k-steps is p = 0: with W >= 2 waves, waves [0, W/2) hold p = 0;
+// one wave keeps a separate accumulator per half.
+
+template <int D, int W>
+struct Mf16ExtLds {
+  static constexpr int TBYTES = Mf16<D>::S * 8;
+  static constexpr int PS = W == 1 ? 2 : W;                 // Gram partial slots (double2 x 256 each)
+  static constexpr int FIN = PS * 256 * 16 + 2 * 256 * 16;  // partials + G_0, G_1
+  static constexpr int BYTES = 2 * TBYTES + (FIN > TBYTES ? FIN - TBYTES : 0);
+};
+
+// This wave's p-resolved Gram partials of bond K into its slot(s) of `part`
+// (after a barrier: `part` overlays Y).
+template <int D, int W, int K>
+__device__ __forceinline__ void gram16_ext(const float2* X, const float2* Y, double2* part) {
+  constexpr int st = cpow16(D - 1 - K);
+  constexpr int KS = Mf16<D>::KS, KPW = KS / W;
+  constexpr int B = KPW < 4 ? KPW : 4;
+  constexpr int NP = W == 1 ? 2 : 1;                      // accumulator sets per wave
+  static_assert(W == 1 || (W % 2 == 0 && KPW <= KS / 2), "one physical index per wave");
+  const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
+  const int gl = opaque(8 * (sw16<D>(fbase16<D, K>(4 * w * KPW)) ^ sw16<D>(fbase16<D, K>(kq) ^ (i16 * st))));
+  d4v gr[NP][2], gi[NP][2];                               // [p][chain]
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) gr[p][c] = gi[p][c] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s0 = 0; s0 < KPW; s0 += B) {
+    float2 xa[B], ya[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int off = gl ^ (8 * sw16<D>(fbase16<D, K>(4 * (s0 + b))));
+      xa[b] = lds_ld2(X, off);
+      ya[b] = lds_ld2(Y, off);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int p = NP == 2 ? (s0 + b) / (KS / 2) : 0;
+      const int c = b & 1;
+      const double xr = xa[b].x, xi = xa[b].y, yr = ya[b].x, yi = ya[b].y;
+      gr[p][c] = mfma4d(xr, yr, gr[p][c]);
+      gi[p][c] = mfma4d(xr, yi, gi[p][c]);
+      gr[p][c] = mfma4d(xi, yi, gr[p][c]);
+      gi[p][c] = mfma4d(-xi, yr, gi[p][c]);
+    }
+  }
+  __syncthreads();                                        // every wave has read Y
+  // result register v of lane l: entry [c = l / 16 + 4 v][e = l % 16]
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int slot = NP == 2 ? p : w;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      part[slot * 256 + (kq + 4 * v) * 16 + i16] =
+          make_double2(gr[p][0][v] + gr[p][1][v], gi[p][0][v] + gi[p][1][v]);
+  }
+}
+
+// Sum the partial slots into G_0, G_1 and write the 32 x 32 extended message of bond K.
+template <int D, int W, int K>
+__device__ __forceinline__ void ext_finish16(double2* part, const ClassView& cv, int i, cplx<double>* ext_out) {
+  constexpr int PS = Mf16ExtLds<D, W>::PS, H = PS / 2, NT = 64 * W;
+  double2* G = part + PS * 256;                           // G_0 at [0, 256), G_1 at [256, 512)
+  __syncthreads();
+  for (int o = threadIdx.x; o < 512; o += NT) {
+    const int p = o >> 8, x = o & 255;
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int q = 0; q < H; ++q) {
+      re += part[(p * H + q) * 256 + x].x;
+      im += part[(p * H + q) * 256 + x].y;
+    }
+    G[o] = make_double2(re, im);
+  }
+  __syncthreads();
+  cplx<double> tr = mk(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const double2 a = G[c * 17], b = G[256 + c * 17];
+    tr = tr + mk(a.x + b.x, a.y + b.y);
+  }
+  const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[K * cv.n + i];
+  const cplx<double> g0 = zg[0], g1 = zg[1];
+  const cplx<double> inv = cinv(tr * mk(abs2(g0) + abs2(g1), 0.0));
+  cplx<double>* out = ext_out + (size_t)cv.out_pos[K * cv.n + i] * 1024;
+  for (int o = threadIdx.x; o < 1024; o += NT) {
+    const int row = o >> 5, col = o & 31;
+    const int s = row >> 4, c = row & 15, s2 = col >> 4, e = col & 15;
+    const double2 a = G[c * 16 + e], b = G[256 + c * 16 + e];
+    const cplx<double> gsum = ((s + s2) & 1) ? mk(a.x - b.x, a.y - b.y) : mk(a.x + b.x, a.y + b.y);
+    out[o] = conj(s ? g1 : g0) * (s2 ? g1 : g0) * gsum * inv;
+  }
+  __syncthreads();                                        // the next mode product overwrites the work buffer
+}
+
+template <int D, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma16_ext(ClassView cv, const cplx<float>* __restrict__ T,
+                                                       const cplx<float>* __restrict__ msgs,
+                                                       cplx<double>* __restrict__ ext_out) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  static_assert(Mf16<D>::TILES % W == 0 && Mf16<D>::KS % W == 0, "tiles and k-steps split evenly over the waves");
+  static_assert(Mf16<D>::S % (128 * W) == 0, "tensor load in whole rounds of float4");
+  extern __shared__ __align__(16) unsigned char smem16[];
+  float2* Ts = reinterpret_cast<float2*>(smem16);
+  float2* Wb = Ts + Mf16<D>::S;
+  double2* part = reinterpret_cast<double2*>(Wb);
+  const int n = cv.n;
+  constexpr int S = Mf16<D>::S;
+  constexpr int NT = 64 * W;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const float4* Tg = reinterpret_cast<const float4*>(T + (size_t)i * S);
+    float4* T4 = reinterpret_cast<float4*>(Ts);
+#pragma unroll
+    for (int k = 0; k < S / (2 * NT); ++k) {
+      const int q = threadIdx.x + NT * k;
+      const float4 v = Tg[q];
+      const int sw = sw16<D>(2 * q);
+      T4[sw >> 1] = (sw & 1) ? make_float4(v.z, v.w, v.x, v.y) : v;
+    }
+    const cplx<float>* m[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) m[j] = msgs + (size_t)cv.in_pos[j * n + i] * 256;
+    __syncthreads();
+    if constexpr (D == 2) {
+      mp16<D, W, 1>(Ts, Wb, m[1]);                     // W = T x1 m1
+      __syncthreads();
+      gram16_ext<D, W, 0>(Ts, Wb, part);
+      ext_finish16<D, W, 0>(part, cv, i, ext_out);
+      mp16<D, W, 0>(Ts, Wb, m[0]);                     // W = T x0 m0
+      __syncthreads();
+      gram16_ext<D, W, 1>(Ts, Wb, part);
+      ext_finish16<D, W, 1>(part, cv, i, ext_out);
+    } else {
+      mp16<D, W, 2>(Ts, Wb, m[2]);                     // W = T x2 m2
+      __syncthreads();
+      mp16<D, W, 1>(Wb, Wb, m[1]);                     // W = W x1 m1
+      __syncthreads();
+      gram16_ext<D, W, 0>(Ts, Wb, part);
+      ext_finish16<D, W, 0>(part, cv, i, ext_out);
+      mp16<D, W, 2>(Ts, Wb, m[2]);                     // W = T x2 m2
+      __syncthreads();
+      mp16<D, W, 0>(Wb, Wb, m[0]);                     // W = W x0 m0
+      __syncthreads();
+      gram16_ext<D, W, 1>(Ts, Wb, part);
+      ext_finish16<D, W, 1>(part, cv, i, ext_out);
+      mp16<D, W, 0>(Ts, Wb, m[0]);                     // W = T x0 m0
+      __syncthreads();
+      mp16<D, W, 1>(Wb, Wb, m[1]);                     // W = W x1 m1
+      __syncthreads();
+      gram16_ext<D, W, 2>(Ts, Wb, part);
+      ext_finish16<D, W, 2>(part, cv, i, ext_out);
+    }
+  }
+}
+
 }  // namespace bqa

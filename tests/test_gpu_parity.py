@@ -529,3 +529,40 @@ def test_chi16_mfma_bp_matches_generic(name):
     finally:
         for job in jobs:
             job.close()
+
+
+def test_chi16_ext_matches_generic():
+    """The chi = 16 FP32 extended-message kernel (FP32 MFMA mode products, FP64 MFMA Grams) against
+    the generic FP64 kernel on one uploaded state at bond dimension 16, through the simple update:
+    chi' and the sorted lambdas, then one fused Trotter step's densities."""
+    cfg, _ = load_golden("heavyhex_chi16")
+    ctx_o = O.make_context(cfg)
+    st_o = O.init_state(ctx_o)
+    layers = [i for i in ctx_o.instructions if isinstance(i, dict)]
+    k = 0
+    while st_o.chi < 16 and k < len(layers) - 1:
+        O.run_layer(ctx_o, st_o, layers[k]["xtime"], layers[k]["ztime"])
+        k += 1
+    assert st_o.chi == 16
+    print(f"[gpu] chi16 ext fast vs generic at step {k}", flush=True)
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, "single", g) for g in (False, True)]
+    try:
+        su = [job.simple_update(layers[k]["ztime"], ctx_o.pinv_eps) for job in jobs]
+        lm = [np.sort(job.get_lambdas().real, axis=1) for job in jobs]
+        print(f"[gpu]   simple update chi' {su[0][0]} / {su[1][0]}, |lambda fast - generic| "
+              f"{np.abs(lm[0] - lm[1]).max():.2e}", flush=True)
+        assert su[0][0] == su[1][0]
+        assert np.abs(lm[0] - lm[1]).max() < 1e-5
+    finally:
+        for job in jobs:
+            job.close()
+    jobs = [_job_from_oracle(cfg, ctx_o, st_o, "single", g) for g in (False, True)]
+    try:
+        for job in jobs:
+            job.run_layer(layers[k]["xtime"], layers[k]["ztime"], ctx_o.pinv_eps, 60, 1e-9, 0.0)
+        r = [job.density()[0] for job in jobs]
+        print(f"[gpu]   fused step |rho fast - generic| {np.abs(r[0] - r[1]).max():.2e}", flush=True)
+        assert np.abs(r[0] - r[1]).max() < 2e-5
+    finally:
+        for job in jobs:
+            job.close()
