@@ -1265,10 +1265,10 @@ size_t estimate(int N, int M, const std::vector<int>& deg, const std::vector<int
   b += (double)M * n2 * ccols * sizeof(zd);                        // canonicalisers
   b += E * n2 * sizeof(double);                                    // lambdas
   b += E * 2.0 * 16.0;                                             // ZZ-split gates
-  // canonicaliser workspace: per edge for the chi <= 8 kernels (largest at chi = min(chimax, 8)),
-  // per workgroup for the generic kernel once its workspace leaves LDS
-  const int cr = std::min(chimax, 8);
-  const int nr = 2 * cr;
+  // canonicaliser workspace: per edge for the split-column kernels (largest at n = 32 once chi
+  // reaches 16, else at chi = min(chimax, 8)), per workgroup for the generic kernel once its
+  // workspace leaves LDS
+  const int nr = chimax >= 16 ? 32 : 2 * std::min(chimax, 8);
   double cws = E * (10.0 * nr * nr + 5.0 * nr) * sizeof(double);
   if (chimax > 8) {
     const size_t wsb = canon_ws_bytes(n2);

@@ -8,6 +8,7 @@
 #include "kernels_node.hpp"
 #include "kernels_mfma.hpp"
 #include "kernels_mfma16.hpp"
+#include "kernels_local.hpp"
 #include "kernels_small.hpp"
 #include "kernels_d4.hpp"
 
@@ -55,9 +56,6 @@ bool fast_density_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>*
   return launch_small<kModeDensity, R, R>(cv, chi, T, m, m, nullptr, nullptr, rho, bloch, s);
 }
 
-// Extended messages in FP64 from R storage: MFMA for chi = 8 (degree 2, 3),
-// k_small for the other served shapes; false leaves the generic kernel.  The
-// ZZ-split gates of the step come from cv.zz.
 // FP32, chi = 16, degree 2 or 3: k_mfma16_ext, W waves per node (128 KB of LDS at degree 3).
 template <int D, int W>
 inline void launch_mfma16_ext(ClassView cv, const cplx<float>* T, const cplx<float>* msgs, cplx<double>* ext,
@@ -67,6 +65,9 @@ inline void launch_mfma16_ext(ClassView cv, const cplx<float>* T, const cplx<flo
   hipLaunchKernelGGL((k_mfma16_ext<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, msgs, ext);
 }
 
+// Extended messages in FP64 from R storage: MFMA for chi = 8 (degree 2, 3) and for FP32
+// jobs at chi = 16 (degree 2, 3), the degree-4 chi = 8 kernel, k_small for the other served
+// shapes; false leaves the generic kernel.  The ZZ-split gates of the step come from cv.zz.
 template <typename R>
 bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext,
                      hipStream_t s) {
@@ -329,12 +330,15 @@ inline bool fast_density_launch<float>(ClassView cv, int chi, const cplx<float>*
   return true;
 }
 
-// chi = 8, degree 3 (FP32 and FP64 storage): k_mfma8_local64; other shapes as above.
+// chi = 8, degree 3 (FP32 and FP64 storage): k_mfma8_local64; chi = 16 at degrees 2-3 and
+// chi = 8 at degree 4: k_local_lds; other shapes as above.
 template <typename R>
 inline bool fast_local_mfma(ClassView cv, int chi_in, int chi_out, const cplx<R>* Tin, cplx<R>* Tout,
                             const cplx<double>* canon, int rows, int cols, double zz_time, int flags, double z_time,
                             double x_time, const double* lmbd, int lstride, cplx<R>* msgs_init, hipStream_t s) {
   if (chi_in != chi_out || flags != (kAbsorb | kGates | kGauge) || cv.zz == nullptr) return false;
+  if (launch_local_big<R>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init, s))
+    return true;
   if (chi_in != 8 || cv.d != 3 || cv.n == 0)
     return launch_small_local<R>(cv, chi_in, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init,
                                  s);

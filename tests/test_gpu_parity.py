@@ -534,7 +534,8 @@ def test_chi16_mfma_bp_matches_generic(name):
 def test_chi16_ext_matches_generic():
     """The chi = 16 FP32 extended-message kernel (FP32 MFMA mode products, FP64 MFMA Grams) against
     the generic FP64 kernel on one uploaded state at bond dimension 16, through the simple update:
-    chi' and the sorted lambdas, then one fused Trotter step's densities."""
+    chi' and the sorted lambdas; then one fused Trotter step (which also runs the chi = 16
+    absorb/gates/gauge kernel) in both precisions, its densities against the generic kernels'."""
     cfg, _ = load_golden("heavyhex_chi16")
     ctx_o = O.make_context(cfg)
     st_o = O.init_state(ctx_o)
@@ -556,13 +557,15 @@ def test_chi16_ext_matches_generic():
     finally:
         for job in jobs:
             job.close()
-    jobs = [_job_from_oracle(cfg, ctx_o, st_o, "single", g) for g in (False, True)]
-    try:
-        for job in jobs:
-            job.run_layer(layers[k]["xtime"], layers[k]["ztime"], ctx_o.pinv_eps, 60, 1e-9, 0.0)
-        r = [job.density()[0] for job in jobs]
-        print(f"[gpu]   fused step |rho fast - generic| {np.abs(r[0] - r[1]).max():.2e}", flush=True)
-        assert np.abs(r[0] - r[1]).max() < 2e-5
-    finally:
-        for job in jobs:
-            job.close()
+    # one fused Trotter step (chi = 16 absorb + gates + gauge kernel in both precisions)
+    for precision, tol in (("single", 2e-5), ("double", 1e-10)):
+        jobs = [_job_from_oracle(cfg, ctx_o, st_o, precision, g) for g in (False, True)]
+        try:
+            for job in jobs:
+                job.run_layer(layers[k]["xtime"], layers[k]["ztime"], ctx_o.pinv_eps, 60, 1e-9, 0.0)
+            r = [job.density()[0] for job in jobs]
+            print(f"[gpu]   fused step {precision} |rho fast - generic| {np.abs(r[0] - r[1]).max():.2e}", flush=True)
+            assert np.abs(r[0] - r[1]).max() < tol
+        finally:
+            for job in jobs:
+                job.close()
