@@ -33,23 +33,34 @@ struct D4Lds {
 };
 
 // In-place mode product along bond J (stride 8^(3-J) within a half): X[.., a, ..] <- sum_b m[a][b] X[.., b, ..].
+// A thread owns 512 / NT fibres and updates them together, so each message entry read from LDS (a
+// broadcast) serves all of them.
 template <int J, int NT, typename C>
 __device__ __forceinline__ void d4_mp(cplx<C>* X, const cplx<C>* __restrict__ m) {
   constexpr int st = J == 0 ? 512 : J == 1 ? 64 : J == 2 ? 8 : 1;
+  constexpr int FPT = 512 / NT;
+  int base[FPT];
+  cplx<C> x[FPT][8];
 #pragma unroll
-  for (int q = 0; q < 512 / NT; ++q) {
-    const int f = threadIdx.x + NT * q;   // fibre
-    const int base = (f / st) * 8 * st + f % st;
-    cplx<C> x[8];
+  for (int u = 0; u < FPT; ++u) {
+    const int f = threadIdx.x + NT * u;   // fibre
+    base[u] = (f / st) * 8 * st + f % st;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) x[b] = X[d4_sw(base + b * st)];
+    for (int b = 0; b < 8; ++b) x[u][b] = X[d4_sw(base[u] + b * st)];
+  }
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      cplx<C> acc = mk<C>(0, 0);
+  for (int a = 0; a < 8; ++a) {
+    cplx<C> acc[FPT];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) cfma(acc, m[a * 8 + b], x[b]);
-      X[d4_sw(base + a * st)] = acc;
+    for (int u = 0; u < FPT; ++u) acc[u] = mk<C>(0, 0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const cplx<C> w = m[a * 8 + b];
+#pragma unroll
+      for (int u = 0; u < FPT; ++u) cfma(acc[u], w, x[u][b]);
     }
+#pragma unroll
+    for (int u = 0; u < FPT; ++u) X[d4_sw(base[u] + a * st)] = acc[u];
   }
 }
 
