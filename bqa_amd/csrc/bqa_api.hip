@@ -1516,8 +1516,11 @@ void batch_bp(bqa_batch* b, int count, bqa_job* const* jobs, const int32_t* max_
     for (int ci = 0; ci < (int)j->cls.size(); ++ci) {
       const ClassDev& cl = j->cls[ci];
       if (cl.d == 0 || cl.n == 0) continue;
-      const int fam = j->generic(kGenBP) ? 0 : j->single() ? multi_family<float>(cl.d, j->chi)
-                                                          : multi_family<double>(cl.d, j->chi);
+      int fam = j->generic(kGenBP) ? 0 : j->single() ? multi_family<float>(cl.d, j->chi)
+                                                    : multi_family<double>(cl.d, j->chi);
+      // autotuned jobs at the k_small_npw shapes share the several-nodes-per-wave kernel (what the
+      // tuner picks for them alone at all but the smallest sizes)
+      if (fam == 1 && npw_shape(cl.d, j->chi) && j->autotune && j->bp_variant_force < 0) fam = kFamNpw;
       if (fam == 0) {
         solo.emplace_back(ri, ci);
         continue;
@@ -1560,7 +1563,7 @@ void batch_bp(bqa_batch* b, int count, bqa_job* const* jobs, const int32_t* max_
         e.mB = j->msgs[1].p;
         e.st = reinterpret_cast<BpStatus*>(j->status.p);
         e.prefix = prefix;
-        prefix += cl.n;
+        prefix += g.fam == kFamNpw ? (cl.n + kNpwShared - 1) / kNpwShared : cl.n;
       }
       g.total = prefix;
       require(g.total <= std::numeric_limits<int>::max(), "too many nodes in one shared launch");

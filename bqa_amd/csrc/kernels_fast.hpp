@@ -255,6 +255,12 @@ inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T,
 // precision R in a launch shared with other jobs -- 1 k_small_multi, 2 k_mfma8_multi, or 0
 // when the class keeps its own launches.  Only shapes whose single-job launch runs the same
 // node code are shared, so a job's results are the same in a batch as alone.
+// Shared-launch families: 1 k_small_multi (variant 0's arithmetic), 2 k_mfma8_multi, kFamNpw
+// k_small_npw_multi with kNpwShared nodes per wave (for jobs whose BP variants are autotuned: the
+// Grams' fibre split, and so the rounding, differs from variant 0), 0 none.
+constexpr int kFamNpw = 3;
+constexpr int kNpwShared = 4;
+
 template <typename R>
 inline int multi_family(int d, int chi) {
   if (sizeof(R) == sizeof(float) && chi == 8 && (d == 2 || d == 3)) return 2;
@@ -289,7 +295,7 @@ inline bool launch_small_multi_d(int chi, const BpEntry* tab, int entries, int t
 }
 
 // One shared launch of sweep `sweep` over `entries` classes (same family, degree and chi)
-// holding `total` nodes; false if the family has no such kernel.
+// holding `total` nodes (node groups for kFamNpw); false if the family has no such kernel.
 template <typename R>
 inline bool launch_bp_multi(int fam, int d, int chi, const BpEntry* tab, int entries, int total, int sweep,
                             hipStream_t s) {
@@ -303,6 +309,20 @@ inline bool launch_bp_multi(int fam, int d, int chi, const BpEntry* tab, int ent
                            entries, sweep);
       return true;
     }
+    return false;
+  }
+  if (fam == kFamNpw) {
+    // total counts node groups of kNpwShared nodes
+#define BQA_NPW_MULTI(DD, CC)                                                                                   \
+  if (d == DD && chi == CC) {                                                                                   \
+    hipLaunchKernelGGL((k_small_npw_multi<DD, CC, kNpwShared, R>),                                              \
+                       dim3(policy_grid<k_small_npw_multi<DD, CC, kNpwShared, R>, 64>(total, 0)), dim3(64), 0, s, \
+                       tab, entries, sweep);                                                                    \
+    return true;                                                                                                \
+  }
+    BQA_NPW_MULTI(2, 2) BQA_NPW_MULTI(2, 3) BQA_NPW_MULTI(2, 4) BQA_NPW_MULTI(3, 2) BQA_NPW_MULTI(3, 3)
+    BQA_NPW_MULTI(3, 4)
+#undef BQA_NPW_MULTI
     return false;
   }
   if (fam != 1) return false;

@@ -495,14 +495,16 @@ __device__ __forceinline__ void sout_g(NpwSmem<D, CHI, NPW, C>& sm, const cplx<C
   small_sync();
 }
 
+// The node groups start, start + gridDim.x, ... (NPW nodes each) of one BP sweep; the
+// block adds its distance maxima only if it had a group (start < groups).
 template <int D, int CHI, int NPW, typename R>
-__global__ __launch_bounds__(64) void k_small_npw(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
-                                                  cplx<R>* msgB, BpStatus* st) {
+__device__ __forceinline__ void npw_run(NpwSmem<D, CHI, NPW, R>& sm, const ClassView& cv,
+                                        const cplx<R>* __restrict__ T, cplx<R>* msgA, cplx<R>* msgB, BpStatus* st,
+                                        long start) {
   static_assert((D == 2 || D == 3) && CHI >= 2 && CHI <= 4, "shape");
   using C = R;
   using M = Sm<D, CHI>;
   constexpr int GL = 64 / NPW;
-  __shared__ __align__(16) NpwSmem<D, CHI, NPW, C> sm;
   const int cur = bp_begin(st, cv.sweep);
   if (cur < 0) return;
   const cplx<R>* min = cur ? msgB : msgA;
@@ -510,7 +512,7 @@ __global__ __launch_bounds__(64) void k_small_npw(ClassView cv, const cplx<R>* _
   const int n = cv.n;
   const int l = threadIdx.x & 63, gi = l / GL, gl = l % GL;
   double dnum = 0.0, dden = 0.0;
-  for (long i0 = (long)blockIdx.x * NPW; i0 < n; i0 += (long)gridDim.x * NPW) {
+  for (long i0 = start * NPW; i0 < n; i0 += (long)gridDim.x * NPW) {
     const int i = (int)i0 + gi;
     const bool valid = i < n;
     const int ic = valid ? i : n - 1;  // a group past the end repeats the last node and stores nothing
@@ -556,7 +558,33 @@ __global__ __launch_bounds__(64) void k_small_npw(ClassView cv, const cplx<R>* _
     dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
     dden = fmax(dden, __shfl_xor(dden, o, 64));
   }
-  if (l == 0) bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
+  if (l == 0 && start * NPW < n) bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
+}
+
+template <int D, int CHI, int NPW, typename R>
+__global__ __launch_bounds__(64) void k_small_npw(ClassView cv, const cplx<R>* __restrict__ T, cplx<R>* msgA,
+                                                  cplx<R>* msgB, BpStatus* st) {
+  __shared__ __align__(16) NpwSmem<D, CHI, NPW, R> sm;
+  npw_run<D, CHI, NPW, R>(sm, cv, T, msgA, msgB, st, blockIdx.x);
+}
+
+// One BP sweep of several jobs' classes (same degree and chi) with NPW nodes per wave: entry e's
+// node group q (NPW nodes) is flat group prefix_e + q (prefix counted in groups), taken by block
+// (prefix_e + q) mod gridDim.x, as in k_small_multi.
+template <int D, int CHI, int NPW, typename R>
+__global__ __launch_bounds__(64) void k_small_npw_multi(const BpEntry* __restrict__ tab, int entries, int sweep) {
+  __shared__ __align__(16) NpwSmem<D, CHI, NPW, R> sm;
+  for (int e = 0; e < entries; ++e) {
+    ClassView cv = tab[e].cv;
+    cv.sweep = sweep;
+    const long g = gridDim.x;
+    const long start = (((long)blockIdx.x - tab[e].prefix) % g + g) % g;
+    if (start * NPW >= cv.n) continue;  // none of this entry's groups falls to this block
+    npw_run<D, CHI, NPW, R>(sm, cv, reinterpret_cast<const cplx<R>*>(tab[e].T),
+                            reinterpret_cast<cplx<R>*>(tab[e].mA), reinterpret_cast<cplx<R>*>(tab[e].mB), tab[e].st,
+                            start);
+    small_sync();  // the next entry's loads overwrite the LDS this one's last readers use
+  }
 }
 
 // Shapes served: degree 2 and 3 with chi 2..8, degree 4 with chi 2..4.

@@ -186,3 +186,29 @@ def test_served_jobs_equal_jobs_alone(batched, precision, monkeypatch):
         alone, steps, _ = run_job(config_to_context(dict(r.config)), precision)
         assert o.steps == steps
         assert np.array_equal(np.array(alone[0][1]), np.array(o.results[0][1])), r.tag
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_autotuned_served_jobs_use_npw_shared_launch(precision, monkeypatch):
+    """With the autotuner on, the batch runs degree-2/3 classes at chi <= 4 through
+    k_small_npw_multi (four nodes per wave): each served job then equals, bit for bit, the job
+    run alone with its BP forced to the four-nodes-per-wave variant (variant 4 at those shapes;
+    the other shapes fall back to variant 0, which the other shared launches compute).  FP64
+    jobs stay at chi <= 4, where every BP launch of the batch is a shared one."""
+    from bqa_amd.config import config_to_context
+    from bqa_amd.serve import run_job
+    monkeypatch.setenv("BQA_AUTOTUNE", "1")
+    monkeypatch.delenv("BQA_BP_VARIANT", raising=False)
+    cap = 8 if precision == "single" else 4
+    mix = ((24, 3, 4), (40, 3, cap), (62, 3, 4), (30, 2, 4), (36, 2, cap), (150, 3, 4))
+    reqs = [Request(_mix_cfg(n, d, c, 70 + k, 10 + 2 * k), arrival=0.0, tag=(n, d, c))
+            for k, (n, d, c) in enumerate(mix)]
+    print(f"[gpu] serving {len(reqs)} autotuned jobs together, batched, {precision}", flush=True)
+    out = serve(reqs, max_concurrent=6, precision=precision, batched=True, batch_workers=1)
+    assert out["failed"] == 0, [o.error for o in out["outcomes"]]
+    monkeypatch.setenv("BQA_BP_VARIANT", "4")
+    for r, o in zip(reqs, out["outcomes"]):
+        alone, steps, _ = run_job(config_to_context(dict(r.config)), precision)
+        assert o.steps == steps
+        assert np.array_equal(np.array(alone[0][1]), np.array(o.results[0][1])), r.tag
