@@ -794,7 +794,7 @@ void launch_canon_h(bqa_job* j, int ccols, double pinv_eps, double deps) {
   LAUNCH_CHECK();
 }
 
-// Register-resident canonicalisers for chi <= 8 and chi = 16 (split-column Cholesky + Jacobi); false
+// Register-resident canonicalisers for chi <= 16 (split-column Cholesky + Jacobi); false
 // for the other bond dimensions.
 bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double deps) {
   switch (chi) {
@@ -806,6 +806,13 @@ bool canon_reg_dispatch(bqa_job* j, int chi, int ccols, double pinv_eps, double 
     case 6: launch_canon_h<12>(j, ccols, pinv_eps, deps); return true;
     case 7: launch_canon_h<14>(j, ccols, pinv_eps, deps); return true;
     case 8: launch_canon_h<16>(j, ccols, pinv_eps, deps); return true;
+    case 9: launch_canon_h<18>(j, ccols, pinv_eps, deps); return true;
+    case 10: launch_canon_h<20>(j, ccols, pinv_eps, deps); return true;
+    case 11: launch_canon_h<22>(j, ccols, pinv_eps, deps); return true;
+    case 12: launch_canon_h<24>(j, ccols, pinv_eps, deps); return true;
+    case 13: launch_canon_h<26>(j, ccols, pinv_eps, deps); return true;
+    case 14: launch_canon_h<28>(j, ccols, pinv_eps, deps); return true;
+    case 15: launch_canon_h<30>(j, ccols, pinv_eps, deps); return true;
     case 16: launch_canon_h<32>(j, ccols, pinv_eps, deps); return true;
     default: return false;
   }
@@ -901,7 +908,7 @@ double canon_deps(const bqa_job* j) {
 }
 
 // n of the split-column canonicaliser serving bond dimension chi, 0 if none does.
-int canon_h_n(int chi) { return chi >= 1 && chi <= 8 ? 2 * chi : chi == 16 ? 32 : 0; }
+int canon_h_n(int chi) { return chi >= 1 && chi <= 16 ? 2 * chi : 0; }
 
 void canon_outputs(bqa_job* j) {
   const int n2 = 2 * j->chi;
@@ -1265,10 +1272,9 @@ size_t estimate(int N, int M, const std::vector<int>& deg, const std::vector<int
   b += (double)M * n2 * ccols * sizeof(zd);                        // canonicalisers
   b += E * n2 * sizeof(double);                                    // lambdas
   b += E * 2.0 * 16.0;                                             // ZZ-split gates
-  // canonicaliser workspace: per edge for the split-column kernels (largest at n = 32 once chi
-  // reaches 16, else at chi = min(chimax, 8)), per workgroup for the generic kernel once its
-  // workspace leaves LDS
-  const int nr = chimax >= 16 ? 32 : 2 * std::min(chimax, 8);
+  // canonicaliser workspace: per edge for the split-column kernels (chi <= 16; largest at chi =
+  // min(chimax, 16)), per workgroup for the generic kernel once its workspace leaves LDS
+  const int nr = 2 * std::min(chimax, 16);
   double cws = E * (10.0 * nr * nr + 5.0 * nr) * sizeof(double);
   if (chimax > 8) {
     const size_t wsb = canon_ws_bytes(n2);
@@ -1375,6 +1381,13 @@ bool canon_multi_dispatch(int n, const CanonEntry* tab, int entries, int blocks,
     case 12: launch_canon_multi<12>(tab, entries, blocks, s); return true;
     case 14: launch_canon_multi<14>(tab, entries, blocks, s); return true;
     case 16: launch_canon_multi<16>(tab, entries, blocks, s); return true;
+    case 18: launch_canon_multi<18>(tab, entries, blocks, s); return true;
+    case 20: launch_canon_multi<20>(tab, entries, blocks, s); return true;
+    case 22: launch_canon_multi<22>(tab, entries, blocks, s); return true;
+    case 24: launch_canon_multi<24>(tab, entries, blocks, s); return true;
+    case 26: launch_canon_multi<26>(tab, entries, blocks, s); return true;
+    case 28: launch_canon_multi<28>(tab, entries, blocks, s); return true;
+    case 30: launch_canon_multi<30>(tab, entries, blocks, s); return true;
     case 32: launch_canon_multi<32>(tab, entries, blocks, s); return true;
     default: return false;
   }

@@ -1,4 +1,4 @@
-// Canonicalisers with split columns for bond dimensions chi <= 8 and chi = 16 (n = 2 chi <= 32).
+// Canonicalisers with split columns for bond dimensions chi <= 16 (n = 2 chi <= 32).
 //
 // Same results as k_canonicalize, organised for occupancy and
 // fewer Jacobi sweeps:
