@@ -305,11 +305,14 @@ def serving_sample(jobs, degrees, chi_cap):
     out["threaded"] = {k: v for k, v in keep(thr).items() if k in pick}
     seq = serve(reqs, max_concurrent=1, batched=False)
     out["one_at_a_time"] = {k: v for k, v in keep(seq).items() if k in pick}
-    out["capacity_over_one_at_a_time"] = round(cap["jobs_per_h"] / max(1e-12, seq["jobs_per_h"]), 3)
-    log(f"[bench] serving d in {degrees}: {out['jobs_per_h']:.4g} jobs/h offered, p50 {out['p50_s']:.3f} s, "
-        f"p99 {out['p99_s']:.3f} s; capacity {cap['jobs_per_h']:.4g} jobs/h (threaded {thr['jobs_per_h']:.4g}); "
-        f"failed {out['failed']}; one at a time {seq['jobs_per_h']:.4g} jobs/h "
-        f"({out['capacity_over_one_at_a_time']:.2f}x)")
+    # a ratio only when every job of both runs completed
+    complete = cap["failed"] == 0 and seq["failed"] == 0 and seq["jobs_per_h"]
+    out["capacity_over_one_at_a_time"] = round(cap["jobs_per_h"] / seq["jobs_per_h"], 3) if complete else None
+    f = lambda v, spec: format(v, spec) if isinstance(v, (int, float)) else "n/a"
+    log(f"[bench] serving d in {degrees}: {f(out['jobs_per_h'], '.4g')} jobs/h offered, p50 {f(out['p50_s'], '.3f')} s, "
+        f"p99 {f(out['p99_s'], '.3f')} s; capacity {f(cap['jobs_per_h'], '.4g')} jobs/h (threaded "
+        f"{f(thr['jobs_per_h'], '.4g')}); failed {out['failed']} / {cap['failed']} / {thr['failed']} / {seq['failed']}; "
+        f"one at a time {f(seq['jobs_per_h'], '.4g')} jobs/h ({f(out['capacity_over_one_at_a_time'], '.2f')}x)")
     return out
 
 
