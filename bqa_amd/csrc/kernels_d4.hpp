@@ -65,16 +65,18 @@ __device__ __forceinline__ void d4_mp(cplx<C>* X, const cplx<C>* __restrict__ m)
 }
 
 // Gram along bond K: g += sum over this thread's fibre group r of conj(A[r, c]) B[r, e], entry (c, e) =
-// (threadIdx % 64) / 8, % 8; the NT / 64 groups of 64 threads take every (NT / 64)-th fibre.
-template <int K, int NT, typename C>
+// (threadIdx % 64) / 8, % 8; the NT / 64 groups of 64 threads take every (NT / 64)-th fibre.  The
+// products and sums are formed in GC (double for the extended messages of FP32 jobs: exact products
+// of the FP32 operands, FP64 sums).
+template <int K, int NT, typename C, typename GC = C>
 __device__ __forceinline__ void d4_gram(const cplx<C>* A, const cplx<C>* B, cplx<double>& g) {
   constexpr int st = K == 0 ? 512 : K == 1 ? 64 : K == 2 ? 8 : 1;
   const int o = threadIdx.x & 63, grp = threadIdx.x >> 6, c = o >> 3, e = o & 7;
-  cplx<C> acc = mk<C>(0, 0);
+  cplx<GC> acc = mk<GC>(0, 0);
 #pragma unroll 8
   for (int r = grp; r < 512; r += NT / 64) {
     const int base = (r / st) * 8 * st + r % st;
-    cfma_cj(acc, A[d4_sw(base + c * st)], B[d4_sw(base + e * st)]);
+    cfma_cj(acc, cast<GC>(A[d4_sw(base + c * st)]), cast<GC>(B[d4_sw(base + e * st)]));
   }
   g = g + cast<double>(acc);
 }
@@ -84,7 +86,7 @@ __device__ __forceinline__ void d4_sync() { __syncthreads(); }
 // One physical half T_p of a node: the schedule above, adding the half's four Grams into g[k].
 // M holds the four incoming messages; on return A and B are still being read by other threads'
 // last Gram (the caller synchronises before reusing them).
-template <int NT, typename R, typename C>
+template <int NT, typename R, typename C, typename GC = C>
 __device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M, const cplx<R>* __restrict__ Tp,
                                         cplx<double> (&g)[4]) {
   constexpr int H = D4Lds<C>::H;
@@ -102,13 +104,13 @@ __device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M
   d4_sync();
   d4_mp<1, NT, C>(A, M + 1 * 64);   // A = T x1 m1
   d4_sync();
-  d4_gram<0, NT, C>(A, B, g[0]);
+  d4_gram<0, NT, C, GC>(A, B, g[0]);
   d4_sync();
   load(A);
   d4_sync();
   d4_mp<0, NT, C>(A, M + 0 * 64);   // A = T x0 m0
   d4_sync();
-  d4_gram<1, NT, C>(A, B, g[1]);
+  d4_gram<1, NT, C, GC>(A, B, g[1]);
   d4_sync();
   load(B);
   d4_sync();
@@ -119,13 +121,13 @@ __device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M
   d4_sync();
   d4_mp<3, NT, C>(A, M + 3 * 64);   // A = T x3 m3
   d4_sync();
-  d4_gram<2, NT, C>(A, B, g[2]);
+  d4_gram<2, NT, C, GC>(A, B, g[2]);
   d4_sync();
   load(A);
   d4_sync();
   d4_mp<2, NT, C>(A, M + 2 * 64);   // A = T x2 m2
   d4_sync();
-  d4_gram<3, NT, C>(A, B, g[3]);
+  d4_gram<3, NT, C, GC>(A, B, g[3]);
 }
 
 // NT threads per node (256 or 512: one or two waves per SIMD, the node filling the CU's LDS in FP64).
@@ -206,8 +208,10 @@ inline void launch_bp_d4c8(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>*
 }
 
 
-// Extended messages of degree-4 nodes at chi = 8 (state.py:127-139, backends.py:393-404,519-526), FP64
-// arithmetic on R storage: the same products per physical half, the Grams kept per half (p), and each
+// Extended messages of degree-4 nodes at chi = 8 (state.py:127-139, backends.py:393-404,519-526): the
+// mode products in the storage precision R and the Grams in FP64 (for FP32 jobs, exact products of the
+// FP32 operands with FP64 sums, as the chi = 8 MFMA kernels do), the same products per physical half,
+// the Grams kept per half (p), and each
 // 16 x 16 extended message assembled from the two halves and the bond's ZZ-split gates g0, g1:
 // ext[(s,c),(s',e)] = conj(g_s) g_s' (G_0 +- G_1)[c][e] / (trace * (|g0|^2 + |g1|^2)), with + for
 // s + s' even.  One node per block, NT threads.
@@ -215,7 +219,7 @@ template <typename R, int NT>
 __global__ __launch_bounds__(NT) void k_ext_d4c8(ClassView cv, const cplx<R>* __restrict__ T,
                                                  const cplx<R>* __restrict__ msgs, cplx<double>* ext_out) {
   static_assert(NT == 256 || NT == 512, "four or eight waves per node");
-  using C = double;
+  using C = R;
   constexpr int H = D4Lds<C>::H, NG = NT / 64;
   extern __shared__ __align__(16) unsigned char d4_lds[];
   cplx<C>* A = reinterpret_cast<cplx<C>*>(d4_lds);
@@ -234,11 +238,12 @@ __global__ __launch_bounds__(NT) void k_ext_d4c8(ClassView cv, const cplx<R>* __
     for (int p = 0; p < 2; ++p)
 #pragma unroll
       for (int k = 0; k < 4; ++k) g[p][k] = mk(0.0, 0.0);
-    d4_half<NT, R, C>(A, B, M, T + (size_t)i * 2 * H, g[0]);
+    d4_half<NT, R, C, double>(A, B, M, T + (size_t)i * 2 * H, g[0]);
     d4_sync();
-    d4_half<NT, R, C>(A, B, M, T + (size_t)i * 2 * H + H, g[1]);
+    d4_half<NT, R, C, double>(A, B, M, T + (size_t)i * 2 * H + H, g[1]);
     d4_sync();
-    cplx<double>* red = A;  // [k][p][grp][64]: at most 64 KB, the A buffer
+    // [k][p][grp][64]: at most 64 KB, the dead A and B buffers (contiguous, 64 KB in FP32)
+    cplx<double>* red = reinterpret_cast<cplx<double>*>(A);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -269,7 +274,7 @@ __global__ __launch_bounds__(NT) void k_ext_d4c8(ClassView cv, const cplx<R>* __
 
 template <typename R, int NT>
 inline void launch_ext_d4c8(ClassView cv, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext, hipStream_t s) {
-  constexpr int LDS = D4Lds<double>::BYTES;
+  constexpr int LDS = D4Lds<R>::BYTES;
   const int grid = node_grid_lds<k_ext_d4c8<R, NT>, NT, LDS>(cv.n, 1);
   hipLaunchKernelGGL((k_ext_d4c8<R, NT>), dim3(grid), dim3(NT), LDS, s, cv, T, msgs, ext);
 }
