@@ -15,6 +15,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "kernels_mfma.hpp"
 #include "kernels_node.hpp"
 #include "launch_grid.hpp"
 
@@ -277,6 +278,207 @@ inline void launch_ext_d4c8(ClassView cv, const cplx<R>* T, const cplx<R>* msgs,
   constexpr int LDS = D4Lds<R>::BYTES;
   const int grid = node_grid_lds<k_ext_d4c8<R, NT>, NT, LDS>(cv.n, 1);
   hipLaunchKernelGGL((k_ext_d4c8<R, NT>), dim3(grid), dim3(NT), LDS, s, cv, T, msgs, ext);
+}
+
+// ---------------------------------------------------------------------------
+// The same sweep for FP32 jobs on v_mfma_f32_16x16x4f32 (k_bp_d4c8m; four waves per node, same
+// schedule, LDS layout and swizzle).  A mode product along bond J is, per tile of 16 fibres, the
+// real GEMM [Yr | Yi] = [Xr | Xi] [[m_r^T, m_i^T], [-m_i^T, m_r^T]] (K = 16: the bond index with
+// its component; N = 16: the output index with its component), four k-steps; a Gram along bond K
+// is one 16 x 16 tile D[(comp, c)][(comp', e)] = sum over fibres of A[f][c].comp B[f][e].comp'
+// (K = the 512 fibres of the half, four per k-step), from which
+//   Gram[c][e] = D[(0,c)][(0,e)] + D[(1,c)][(1,e)] + i (D[(0,c)][(1,e)] - D[(1,c)][(0,e)]).
+// Operand lane layouts: A[i][k] at lane i + 16 k, B[k][j] at lane j + 16 k, D[4 (l / 16) + v][l % 16]
+// in result register v of lane l.  Mode products read a tile's operands, then write its results in
+// place (a wave's tiles are disjoint fibres); each wave's Gram partials stay in FP32 MFMA chains for
+// one half and are added in FP64.
+
+// In-place mode product along bond J with message m (64 complex entries, m[a][b] at a * 8 + b).
+template <int J>
+__device__ __forceinline__ void d4m_mp(cplx<float>* X, const cplx<float>* m) {
+  constexpr int st = J == 0 ? 512 : J == 1 ? 64 : J == 2 ? 8 : 1;
+  constexpr int TPW = 8;  // 32 tiles of 16 fibres per half, four waves
+  // the lane's indices through opaque(): its addresses are formed here, not hoisted out of the node loop
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, w = tid >> 6, i16 = l & 15, kq = l >> 4;
+  const int a = i16 & 7, co = i16 >> 3;
+  float bm[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * s + kq, ci = k >> 3, b = k & 7;
+    const cplx<float> mv = m[a * 8 + b];
+    bm[s] = ci == co ? mv.x : (ci ? -mv.y : mv.y);
+  }
+  float xa[TPW][4];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int f = (w * TPW + t) * 16 + i16;
+    const int base = (f / st) * 8 * st + f % st;
+    const cplx<float> x0 = X[d4_sw(base + kq * st)], x1 = X[d4_sw(base + (kq + 4) * st)];
+    xa[t][0] = x0.x;
+    xa[t][1] = x1.x;
+    xa[t][2] = x0.y;
+    xa[t][3] = x1.y;
+  }
+  f4v acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = mfma4(xa[t][s], bm[s], acc[t]);
+  float* Xf = reinterpret_cast<float*>(X);
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int f = (w * TPW + t) * 16 + 4 * kq + v;
+      const int base = (f / st) * 8 * st + f % st;
+      Xf[2 * d4_sw(base + a * st) + co] = acc[t][v];
+    }
+}
+
+// This wave's k-steps of the Gram tile along bond K, added (FP64) into g[v] = D[4 (l / 16) + v][l % 16].
+template <int K>
+__device__ __forceinline__ void d4m_gram(const cplx<float>* A, const cplx<float>* B, double (&g)[4]) {
+  constexpr int st = K == 0 ? 512 : K == 1 ? 64 : K == 2 ? 8 : 1;
+  constexpr int KPW = 32;  // 128 k-steps of four fibres per half, four waves
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, w = tid >> 6, x = l & 7, comp = (l >> 3) & 1, kq = l >> 4;
+  f4v acc[4] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f},
+                f4v{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll 4
+  for (int s = 0; s < KPW; ++s) {
+    const int f = 4 * (w * KPW + s) + kq;
+    const int idx = d4_sw((f / st) * 8 * st + f % st + x * st);
+    const cplx<float> av = A[idx], bv = B[idx];
+    acc[s & 3] = mfma4(comp ? av.y : av.x, comp ? bv.y : bv.x, acc[s & 3]);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    g[v] += ((double)acc[0][v] + (double)acc[1][v]) + ((double)acc[2][v] + (double)acc[3][v]);
+}
+
+// One physical half: d4_half's schedule on the MFMA products.
+__device__ __forceinline__ void d4m_half(cplx<float>* A, cplx<float>* B, const cplx<float>* M,
+                                         const cplx<float>* __restrict__ Tp, double (&g)[4][4]) {
+  constexpr int H = D4Lds<float>::H, NT = 256;
+  auto load = [&](cplx<float>* X) {
+    const int t = opaque((int)threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < H / NT; ++q) X[d4_sw(t + NT * q)] = Tp[t + NT * q];
+  };
+  load(B);
+  d4_sync();
+  d4m_mp<2>(B, M + 2 * 64);
+  d4_sync();
+  d4m_mp<3>(B, M + 3 * 64);  // B = T x2 m2 x3 m3
+  load(A);
+  d4_sync();
+  d4m_mp<1>(A, M + 1 * 64);  // A = T x1 m1
+  d4_sync();
+  d4m_gram<0>(A, B, g[0]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4m_mp<0>(A, M + 0 * 64);  // A = T x0 m0
+  d4_sync();
+  d4m_gram<1>(A, B, g[1]);
+  d4_sync();
+  load(B);
+  d4_sync();
+  d4m_mp<0>(B, M + 0 * 64);
+  d4_sync();
+  d4m_mp<1>(B, M + 1 * 64);  // B = T x0 m0 x1 m1
+  load(A);
+  d4_sync();
+  d4m_mp<3>(A, M + 3 * 64);  // A = T x3 m3
+  d4_sync();
+  d4m_gram<2>(A, B, g[2]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4m_mp<2>(A, M + 2 * 64);  // A = T x2 m2
+  d4_sync();
+  d4m_gram<3>(A, B, g[3]);
+}
+
+__global__ __launch_bounds__(256, 2) void k_bp_d4c8m(ClassView cv, const cplx<float>* __restrict__ T,
+                                                  cplx<float>* msgA, cplx<float>* msgB, BpStatus* st) {
+  constexpr int H = D4Lds<float>::H, NW = 4;
+  extern __shared__ __align__(16) unsigned char d4_lds[];
+  cplx<float>* A = reinterpret_cast<cplx<float>*>(d4_lds);
+  cplx<float>* B = A + H;
+  cplx<float>* M = B + H;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const cplx<float>* min = cur ? msgB : msgA;
+  cplx<float>* mout = cur ? msgA : msgB;
+  const int n = cv.n;
+  const int t = threadIdx.x, o = t & 63;
+  double dnum = 0.0, dden = 0.0;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    d4_sync();  // the previous node's last readers of A, B and M are done
+    M[(t >> 6) * 64 + o] = min[(size_t)cv.in_pos[(t >> 6) * n + i] * 64 + o];
+    double g[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g[k][v] = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      if (p) d4_sync();  // the last Gram of the first half has read B
+      d4m_half(A, B, M, T + (size_t)i * 2 * H + (size_t)p * H, g);
+    }
+    // the waves' Gram tiles into the dead A buffer (FP64, [k][wave][256]: 32 KB), then one wave finishes
+    d4_sync();
+    double* red = reinterpret_cast<double*>(A);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[(k * NW + (t >> 6)) * 256 + (4 * (o >> 4) + v) * 16 + (o & 15)] = g[k][v];
+    d4_sync();
+    if (t < 64) {
+      const int c = o >> 3, e = o & 7;
+#pragma unroll 1
+      for (int k = 0; k < 4; ++k) {
+        auto dsum = [&](int r, int col) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NW; ++q) s += red[(k * NW + q) * 256 + r * 16 + col];
+          return s;
+        };
+        auto entry = [&](int cc, int ee) {
+          return mk(dsum(cc, ee) + dsum(8 + cc, 8 + ee), dsum(cc, 8 + ee) - dsum(8 + cc, ee));
+        };
+        double tr = 0.0;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) tr += dsum(d, d) + dsum(8 + d, 8 + d);
+        const double inv = 1.0 / tr;
+        const cplx<double> a = entry(c, e), b = entry(e, c);
+        const cplx<float> v = cast<float>(mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv));
+        const size_t pos = (size_t)cv.out_pos[k * n + i];
+        const cplx<double> vd = cast<double>(v), od = cast<double>(min[pos * 64 + o]);
+        dnum = fmax(dnum, abs2(vd - od));
+        dden = fmax(dden, abs2(vd + od));
+        mout[pos * 64 + o] = v;
+      }
+    }
+  }
+  if (t < 64) {
+    for (int s = 32; s > 0; s >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, s, 64));
+      dden = fmax(dden, __shfl_xor(dden, s, 64));
+    }
+    if (t == 0) bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
+  }
+}
+
+inline void launch_bp_d4c8m(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
+                            hipStream_t s) {
+  constexpr int LDS = D4Lds<float>::BYTES;
+  const int grid = node_grid_lds<k_bp_d4c8m, 256, LDS>(cv.n, 1);
+  hipLaunchKernelGGL(k_bp_d4c8m, dim3(grid), dim3(256), LDS, s, cv, T, mA, mB, st);
 }
 
 }  // namespace bqa

@@ -122,13 +122,13 @@ inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA
 //   chi = 16, d = 3: k_mfma16_bp<3, W>, W = 8 or 4 waves per node (one node per CU, 128 KB LDS);
 //   chi = 16, d = 2: k_mfma16_bp<2, 1>, one or two resident rounds;
 //   chi = 8, d = 2, 3: k_mfma8 (one wave per node) under the three grid policies;
-//   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node;
+//   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node, k_bp_d4c8m (MFMA);
 //   other served shapes: k_small under the three grid policies, and for degree 2, 3 with chi <= 4
 //   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<float>(int d, int chi) {
   if (chi == 16 && (d == 2 || d == 3)) return 2;
-  if (chi == 8 && d == 4) return 2;
+  if (chi == 8 && d == 4) return 3;
   if (chi == 8 && (d == 2 || d == 3)) return kGridPolicies;
   if (npw_shape(d, chi)) return kGridPolicies + 2;
   return small_bp_shape<float>(d, chi) ? kGridPolicies : 1;
@@ -138,7 +138,8 @@ template <>
 inline const char* bp_variant_name<float>(int d, int chi, int v) {
   if (chi == 16 && d == 3) return v == 0 ? "k_mfma16_bp<3,8>" : "k_mfma16_bp<3,4>";
   if (chi == 16 && d == 2) return v == 0 ? "k_mfma16_bp<2,1>, resident" : "k_mfma16_bp<2,1>, 2x resident";
-  if (chi == 8 && d == 4) return v == 0 ? "k_bp_d4c8<256 threads>" : "k_bp_d4c8<512 threads>";
+  if (chi == 8 && d == 4)
+    return v == 0 ? "k_bp_d4c8<256 threads>" : v == 1 ? "k_bp_d4c8<512 threads>" : "k_bp_d4c8m (FP32 MFMA)";
   static const char* m8[kGridPolicies] = {"k_mfma8, resident SIMD-rounded", "k_mfma8, resident",
                                           "k_mfma8, 2x resident"};
   static const char* sm[kGridPolicies + 2] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident",
@@ -175,7 +176,9 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
     return v == kGridPolicies ? launch_small_npw<2, float>(cv, chi, T, mA, mB, st, s)
                               : launch_small_npw<4, float>(cv, chi, T, mA, mB, st, s);
   if (chi == 8 && cv.d == 4 && cv.n > 0) {
-    if (v == 1)
+    if (v == 2)
+      launch_bp_d4c8m(cv, T, mA, mB, st, s);
+    else if (v == 1)
       launch_bp_d4c8<float, 512>(cv, T, mA, mB, st, s);
     else
       launch_bp_d4c8<float, 256>(cv, T, mA, mB, st, s);
