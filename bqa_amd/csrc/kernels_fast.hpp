@@ -118,8 +118,18 @@ inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA
   hipLaunchKernelGGL((k_mfma16_bp<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, mA, mB, st);
 }
 
+// FP32, chi = 16, degree 3: k_mfma16_bp3h (three mode products per node, one physical half at a time).
+template <int W>
+inline void launch_mfma16_bp3h(ClassView cv, const cplx<float>* T, cplx<float>* mA, cplx<float>* mB, BpStatus* st,
+                               hipStream_t s) {
+  constexpr int LDS = Mf16HLds<W>::BYTES;
+  const int grid = node_grid_lds<k_mfma16_bp3h<W>, 64 * W, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_mfma16_bp3h<W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, mA, mB, st);
+}
+
 // FP32 BP sweep variants.
-//   chi = 16, d = 3: k_mfma16_bp<3, W>, W = 8 or 4 waves per node (one node per CU, 128 KB LDS);
+//   chi = 16, d = 3: k_mfma16_bp<3, W>, W = 8 or 4 waves per node (one node per CU, 128 KB LDS),
+//   k_mfma16_bp3h<8> (three products per node, 96 KB LDS);
 //   chi = 16, d = 2: k_mfma16_bp<2, 1>, one or two resident rounds;
 //   chi = 8, d = 2, 3: k_mfma8 (one wave per node) under the three grid policies;
 //   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node, k_bp_d4c8m (MFMA);
@@ -127,7 +137,8 @@ inline void launch_mfma16_bp(ClassView cv, const cplx<float>* T, cplx<float>* mA
 //   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<float>(int d, int chi) {
-  if (chi == 16 && (d == 2 || d == 3)) return 2;
+  if (chi == 16 && d == 3) return 3;
+  if (chi == 16 && d == 2) return 2;
   if (chi == 8 && d == 4) return 3;
   if (chi == 8 && (d == 2 || d == 3)) return kGridPolicies;
   if (npw_shape(d, chi)) return kGridPolicies + 2;
@@ -136,7 +147,8 @@ inline int bp_variant_count<float>(int d, int chi) {
 
 template <>
 inline const char* bp_variant_name<float>(int d, int chi, int v) {
-  if (chi == 16 && d == 3) return v == 0 ? "k_mfma16_bp<3,8>" : "k_mfma16_bp<3,4>";
+  if (chi == 16 && d == 3)
+    return v == 0 ? "k_mfma16_bp<3,8>" : v == 1 ? "k_mfma16_bp<3,4>" : "k_mfma16_bp3h<8> (three products)";
   if (chi == 16 && d == 2) return v == 0 ? "k_mfma16_bp<2,1>, resident" : "k_mfma16_bp<2,1>, 2x resident";
   if (chi == 8 && d == 4)
     return v == 0 ? "k_bp_d4c8<256 threads>" : v == 1 ? "k_bp_d4c8<512 threads>" : "k_bp_d4c8m (FP32 MFMA)";
@@ -163,7 +175,9 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
                                   BpStatus* st, hipStream_t s, int v) {
   if (chi == 16 && cv.n > 0 && (cv.d == 2 || cv.d == 3)) {
     if (cv.d == 3) {
-      if (v == 1)
+      if (v == 2)
+        launch_mfma16_bp3h<8>(cv, T, mA, mB, st, s);
+      else if (v == 1)
         launch_mfma16_bp<3, 4>(cv, T, mA, mB, st, s, 1);
       else
         launch_mfma16_bp<3, 8>(cv, T, mA, mB, st, s, 1);

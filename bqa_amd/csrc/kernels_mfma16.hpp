@@ -92,12 +92,14 @@ __device__ __forceinline__ void lds_st2(float2* base, int byte_off, float2 v) {
 // folds at compile time, and each access costs one XOR.
 
 // Y = X x_J m over this wave's tiles (float2 views of the swizzled LDS buffers); X may equal Y.
-template <int D, int W, int J>
+// HALF: X and Y hold one physical half (the first half of the fibres, p being their most
+// significant index), so only its tiles are done.
+template <int D, int W, int J, bool HALF = false>
 __device__ __forceinline__ void mp16(const float2* X, float2* Y, const cplx<float>* __restrict__ m) {
   float bm[8][2];                           // loaded per product: the node's operands do not stay live
   msg_operand16(m, bm);
   constexpr int st = cpow16(D - 1 - J);
-  constexpr int TPW = Mf16<D>::TILES / W;
+  constexpr int TPW = Mf16<D>::TILES / (HALF ? 2 : 1) / W;
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
   const int wbase = sw16<D>(fbase16<D, J>(w * TPW * 16));
   // A reads: fibre row_perm16(i16), element kq along J
@@ -137,10 +139,10 @@ __device__ __forceinline__ void mp16(const float2* X, float2* Y, const cplx<floa
 
 // This wave's k-steps of out[c][e] = sum_f conj(X[f][c]) Y[f][e] over the fibres along bond K:
 // lane (c = 4 kq + v, e = l % 16) gets its four entries, FP32 chains combined in FP64.
-template <int D, int W, int K>
+template <int D, int W, int K, bool HALF = false>
 __device__ __forceinline__ void gram16(const float2* X, const float2* Y, double (&gre)[4], double (&gim)[4]) {
   constexpr int st = cpow16(D - 1 - K);
-  constexpr int KPW = Mf16<D>::KS / W;
+  constexpr int KPW = Mf16<D>::KS / (HALF ? 2 : 1) / W;
   constexpr int B = KPW < 4 ? KPW : 4;
   const int l = threadIdx.x & 63, i16 = l & 15, kq = l >> 4, w = threadIdx.x >> 6;
   // fibre 4 (w KPW + s) + kq, element i16 along K
@@ -305,6 +307,118 @@ __global__ __launch_bounds__(64 * W) void k_mfma16_bp(ClassView cv, const cplx<f
       b = fmax(b, red[1][k]);
     }
     bp_end(st, cv.sweep, sqrt(a), sqrt(b));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The degree-3 sweep with the messages moved to the bra side (k_mfma16_bp3h).  Every message is
+// Hermitian, so sum_a conj(T[..a..]) m[a][b] = conj((T x m)[..b..]); with P_j = T x_j m_j,
+//   out_0 = Gram_0(P_1, P_2),  out_1 = Gram_1(P_0, P_2),  out_2 = Gram_2(P_0, P_1)
+// (bra first): three mode products and three Grams per node instead of six and three.  The three
+// products of a node do not fit in LDS together, so the node runs one physical half at a time (the
+// Grams sum over p): T_p in buffer 0 (32 KB), P_2 and P_1 into buffers 1 and 2, then P_0 in place of
+// T_p; the Gram partials of both halves add up in FP64 registers.  96 KB of LDS, one node per CU.
+template <int W>
+struct Mf16HLds {
+  static constexpr int HB = Mf16<3>::S / 2 * 8;             // bytes of one half buffer
+  static constexpr int FIN = W * 256 * 16 + 256 * 16;       // partials + summed Gram, over the buffers
+  static_assert(FIN <= 3 * HB, "finishing area fits in the three buffers");
+  static constexpr int BYTES = 3 * HB;
+};
+
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_mfma16_bp3h(ClassView cv, const cplx<float>* __restrict__ T,
+                                                        cplx<float>* msgA, cplx<float>* msgB, BpStatus* st) {
+  constexpr int D = 3, S = Mf16<D>::S, SH = S / 2, NT = 64 * W;
+  static_assert(Mf16<D>::TILES % (2 * W) == 0 && Mf16<D>::KS % (2 * W) == 0, "half tiles split over the waves");
+  static_assert(SH % (2 * NT) == 0, "half loads in whole rounds of float4");
+  extern __shared__ __align__(16) unsigned char smem16[];
+  float2* B0 = reinterpret_cast<float2*>(smem16);   // T_p, then P_0 in place
+  float2* B1 = B0 + SH;                             // P_2
+  float2* B2 = B1 + SH;                             // P_1
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const int n = cv.n;
+  const cplx<float>* min = cur ? msgB : msgA;
+  cplx<float>* mout = cur ? msgA : msgB;
+  double dnum = 0.0, dden = 0.0;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, kq = l >> 4, e16 = l & 15;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<float>* m[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) m[j] = min + (size_t)cv.in_pos[j * n + i] * 256;
+    double gr[D][4], gi[D][4];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) gr[k][v] = gi[k][v] = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      __syncthreads();  // the previous half's (or node's) readers of the buffers are done
+      const float4* Tg = reinterpret_cast<const float4*>(T + (size_t)i * S + (size_t)p * SH);
+      float4* T4 = reinterpret_cast<float4*>(B0);
+#pragma unroll
+      for (int k = 0; k < SH / (2 * NT); ++k) {
+        const int q = threadIdx.x + NT * k;
+        const float4 v = Tg[q];
+        const int sw = sw16<D>(2 * q);
+        T4[sw >> 1] = (sw & 1) ? make_float4(v.z, v.w, v.x, v.y) : v;
+      }
+      __syncthreads();
+      mp16<D, W, 2, true>(B0, B1, m[2]);             // P_2
+      mp16<D, W, 1, true>(B0, B2, m[1]);             // P_1
+      __syncthreads();
+      double a[4], b[4];
+      gram16<D, W, 0, true>(B2, B1, a, b);           // out_0 += Gram_0(P_1, P_2)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        gr[0][v] += a[v];
+        gi[0][v] += b[v];
+      }
+      mp16<D, W, 0, true>(B0, B0, m[0]);             // P_0 in place (the Gram above reads B1, B2 only)
+      __syncthreads();
+      gram16<D, W, 1, true>(B0, B1, a, b);           // out_1 += Gram_1(P_0, P_2)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        gr[1][v] += a[v];
+        gi[1][v] += b[v];
+      }
+      gram16<D, W, 2, true>(B0, B2, a, b);           // out_2 += Gram_2(P_0, P_1)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        gr[2][v] += a[v];
+        gi[2][v] += b[v];
+      }
+    }
+    __syncthreads();  // every wave has read the three buffers
+    double2* part = reinterpret_cast<double2*>(B0);
+    double2* Sg = part + W * 256;
+#pragma unroll 1
+    for (int k = 0; k < D; ++k) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part[w * 256 + (4 * kq + v) * 16 + e16] = make_double2(gr[k][v], gi[k][v]);
+      __syncthreads();
+      finish16<W>(part, Sg, cv.out_pos[k * n + i], min, mout, dnum, dden);
+      __syncthreads();
+    }
+  }
+  __shared__ double red[2][W];
+  for (int o = 32; o > 0; o >>= 1) {
+    dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+    dden = fmax(dden, __shfl_xor(dden, o, 64));
+  }
+  if (l == 0) {
+    red[0][w] = dnum;
+    red[1][w] = dden;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+    for (int k = 0; k < W; ++k) {
+      x = fmax(x, red[0][k]);
+      y = fmax(y, red[1][k]);
+    }
+    bp_end(st, cv.sweep, sqrt(x), sqrt(y));
   }
 }
 

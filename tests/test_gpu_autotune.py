@@ -33,7 +33,7 @@ CASES = {
     "rr6000_chi8_fp64": (lambda: instances.regular_uniform(6000, 3, seed=42, field_sigma=0.3), 8, "double", 4, 1e-13),
     "heavyhex_chi8_fp64": (lambda: instances.heavy_hex_gaussian(24, 24, seed=43, field_sigma=0.5), 8, "double", 4,
                            1e-13),
-    "heavyhex_chi16_fp32": (lambda: instances.heavy_hex_gaussian(6, 6, seed=44, field_sigma=0.5), 16, "single", 2,
+    "heavyhex_chi16_fp32": (lambda: instances.heavy_hex_gaussian(6, 6, seed=44, field_sigma=0.5), 16, "single", 3,
                             2e-6),
     # degree 4 at chi = 8: k_bp_d4c8 with 256 or 512 threads per node (another fibre split of the Grams),
     # and for FP32 also k_bp_d4c8m (FP32 MFMA, FP32 Gram chains added in FP64)
