@@ -1,7 +1,7 @@
-// Fused absorb + Z/X gates + symmetric gauge + message re-initialisation for the
-// shapes whose node tensor fills a CU's LDS in FP64: chi = 16 at degrees 2 and 3
-// (the heavy-hex chi = 16 anneal) and chi = 8 at degree 4 (the square grid's
-// interior).  Same operation as k_node_local with all three flags and chi_in =
+// Fused absorb + Z/X gates + symmetric gauge + message re-initialisation for
+// shapes with large node tensors: chi = 16 at degrees 2 and 3 (the heavy-hex
+// chi = 16 anneal), chi = 8 at degree 4 (the square grid's interior) and chi = 4
+// at degree 5 (serving jobs).  Same operation as k_node_local with all three flags and chi_in =
 // chi_out (backends.py:416-432, state.py:142-156, 219-227):
 //
 //   absorb, bond k:  T[p, .., a_k, ..] <- sum_c T[p, .., c, ..] W_p[c][a_k],
@@ -32,7 +32,8 @@ struct LocalShape {
   static constexpr int A = cpow_i(CHI, D);   // elements of one physical half
   static constexpr int S = 2 * A;
   static constexpr int FIB = S / CHI;        // fibres along one bond
-  static constexpr int L = CHI == 16 ? 4 : 3;
+  static constexpr int L = CHI == 16 ? 4 : CHI == 8 ? 3 : 2;   // log2 CHI
+  static_assert(CHI == 16 || CHI == 8 || CHI == 4, "power-of-two bond dimension");
   // LDS: the tensor, one bond's W_0 and W_1, sqrt(lambda) per bond, the norm partials
   static constexpr int BYTES = S * 16 + 2 * CHI * CHI * 16 + D * CHI * 8 + 16 * 8;
 };
@@ -42,7 +43,7 @@ struct LocalShape {
 // (read whole by one thread) falls on different banks across the lanes.
 template <int CHI>
 __device__ __forceinline__ int lsw(int idx) {
-  constexpr int L = CHI == 16 ? 4 : 3;
+  constexpr int L = CHI == 16 ? 4 : CHI == 8 ? 3 : 2;
   return idx ^ ((idx >> L) & (CHI - 1));
 }
 
@@ -154,6 +155,13 @@ __global__ __launch_bounds__(NT) void k_local_lds(ClassView cv, const cplx<R>* _
       __syncthreads();
       local_mp<D, CHI, 3, NT, SPLIT, FPT>(X, Wt);
     }
+    if constexpr (D >= 5) {
+      __syncthreads();
+      local_operand<D, CHI, NT>(Wt, cv, i, 4, canon, canon_rows, canon_cols);
+      __syncthreads();
+      local_mp<D, CHI, 4, NT, SPLIT, FPT>(X, Wt);
+    }
+    static_assert(D >= 2 && D <= 5, "degrees 2 to 5");
     __syncthreads();
     // gates and gauge on the element pairs (a, A + a); element a's bond indices are its
     // base-CHI digits, the last bond lowest
@@ -220,6 +228,8 @@ inline bool launch_local_big(ClassView cv, int chi, const cplx<R>* Tin, cplx<R>*
     launch_local_lds<2, 16, 64, 2, 1, R>(cv, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init, s);
   else if (chi == 8 && cv.d == 4)
     launch_local_lds<4, 8, 512, 1, 2, R>(cv, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init, s);
+  else if (chi == 4 && cv.d == 5)
+    launch_local_lds<5, 4, 256, 1, 2, R>(cv, Tin, Tout, canon, rows, cols, z_time, x_time, lmbd, lstride, msgs_init, s);
   else
     return false;
   return true;

@@ -395,7 +395,8 @@ SMALL_CASES = {
     "heavyhex_chi5": (lambda: instances.heavy_hex_gaussian(2, 3, seed=34, field_sigma=0.5), 5),
     "rr24_chi6": (lambda: instances.regular_uniform(24, 3, seed=35, field_sigma=0.4), 6),
     "rr24_chi7": (lambda: instances.regular_uniform(24, 3, seed=36, field_sigma=0.4), 7),
-    # degree 5: the FP32 BP sweep runs k_small (chi <= 4); FP64 and the other phases run generic kernels
+    # degree 5: the FP32 BP sweep runs k_small (chi <= 4), the fused step at chi = 4 k_local_lds; FP64 BP
+    # and the other phases run generic kernels
     "rr20_d5_chi3": (lambda: instances.regular_uniform(20, 5, seed=37, field_sigma=0.4), 3),
     "rr20_d5_chi4": (lambda: instances.regular_uniform(20, 5, seed=38, field_sigma=0.4), 4),
 }
