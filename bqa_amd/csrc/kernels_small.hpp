@@ -16,6 +16,8 @@
 // double for the extended messages).
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "launch_grid.hpp"
 #include "kernels_node.hpp"
@@ -44,13 +46,14 @@ struct Sm {
   static constexpr bool POW2 = (Q & (Q - 1)) == 0;
 };
 
-template <int D, int CHI, typename C>
+// C: the buffers' element type; GC: the Grams' (double for the extended messages).
+template <int D, int CHI, typename C, typename GC = C>
 struct SmallSmem {
   cplx<C> T[Sm<D, CHI>::S];
   cplx<C> X[Sm<D, CHI>::S];
   cplx<C> Y[Sm<D, CHI>::S];
   cplx<C> m[D][CHI * CHI];
-  cplx<C> red[2][64];
+  cplx<GC> red[2][64];
 };
 
 // Y = X x_J m (m: CHI x CHI in LDS, row a gives output index a along J).
@@ -76,27 +79,28 @@ __device__ __forceinline__ void smp(const cplx<C>* X, cplx<C>* Y, const cplx<C>*
 // PSPLIT separate sums over the p = 0 and p = 1 halves of the fibres (p is the
 // slowest index).  Lane l takes entry o = l % Q and the fibres r = g mod NG,
 // g = l / Q; the partials are combined across g.  On return lanes l < Q hold
-// the full sums g[0] (and g[1] with PSPLIT).
-template <int D, int CHI, int K, bool PSPLIT, typename C>
-__device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<C> (&red)[2][64],
-                                      cplx<C> (&g)[2]) {
+// the full sums g[0] (and g[1] with PSPLIT).  The products and sums are formed in GC.
+template <int D, int CHI, int K, bool PSPLIT, typename C, typename GC = C>
+__device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<GC> (&red)[2][64],
+                                      cplx<GC> (&g)[2]) {
   using M = Sm<D, CHI>;
   constexpr int sk = cpow(CHI, D - 1 - K);
   const int l = threadIdx.x & 63;
   const int o = l % M::Q, grp = l / M::Q;
   const int c = o / CHI, e = o % CHI;
-  g[0] = mk<C>(0, 0);
-  g[1] = mk<C>(0, 0);
+  g[0] = mk<GC>(0, 0);
+  g[1] = mk<GC>(0, 0);
   if (grp < M::NG) {
 #pragma unroll 8
     for (int r0 = 0; r0 < M::RT; r0 += M::NG) {
       const int r = r0 + grp;
       if (M::RT % M::NG == 0 || r < M::RT) {
         const int base = (r / sk) * CHI * sk + r % sk;
+        const cplx<GC> a = cast<GC>(T[base + c * sk]), b = cast<GC>(Y[base + e * sk]);
         if (PSPLIT && r >= M::RT / 2)
-          cfma_cj(g[1], T[base + c * sk], Y[base + e * sk]);
+          cfma_cj(g[1], a, b);
         else
-          cfma_cj(g[0], T[base + c * sk], Y[base + e * sk]);
+          cfma_cj(g[0], a, b);
       }
     }
   }
@@ -116,7 +120,7 @@ __device__ __forceinline__ void sgram(const cplx<C>* T, const cplx<C>* Y, cplx<C
     if (l < M::Q) {
 #pragma unroll
       for (int p = 0; p < (PSPLIT ? 2 : 1); ++p) {
-        cplx<C> s = red[p][l];
+        cplx<GC> s = red[p][l];
 #pragma unroll
         for (int q = 1; q < M::NG; ++q) s = s + red[p][l + q * M::Q];
         g[p] = s;
@@ -183,18 +187,22 @@ __device__ __forceinline__ void sfinish_ext(const cplx<double> (&g)[2], cplx<dou
   small_sync();
 }
 
+// Grams of the extended messages are FP64 whatever the buffers hold.
+template <int MODE, typename C>
+using SmallGC = typename std::conditional<MODE == kModeExt, double, C>::type;
+
 // Outgoing message (BP) or extended message (Ext) along bond K from Y_K.
 template <int D, int CHI, int K, int MODE, typename R, typename C>
-__device__ __forceinline__ void sout(SmallSmem<D, CHI, C>& sm, const cplx<C>* Y, const ClassView& cv, int i,
-                                     const cplx<R> (&old)[D], cplx<R>* mout, cplx<double>* ext_out,
-                                     double& dnum, double& dden) {
-  cplx<C> g[2];
-  sgram<D, CHI, K, MODE == kModeExt, C>(sm.T, Y, sm.red, g);
+__device__ __forceinline__ void sout(SmallSmem<D, CHI, C, SmallGC<MODE, C>>& sm, const cplx<C>* Y,
+                                     const ClassView& cv, int i, const cplx<R> (&old)[D], cplx<R>* mout,
+                                     cplx<double>* ext_out, double& dnum, double& dden) {
+  using GC = SmallGC<MODE, C>;
+  cplx<GC> g[2];
+  sgram<D, CHI, K, MODE == kModeExt, C, GC>(sm.T, Y, sm.red, g);
   const long pos = cv.out_pos[K * cv.n + i];
   if constexpr (MODE == kModeBP) {
     sfinish_bp<CHI, R, C>(g, sm.red, pos, old[K], mout, dnum, dden);
   } else {
-    static_assert(sizeof(C) == sizeof(double), "extended messages in FP64");
     const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[K * cv.n + i];
     sfinish_ext<CHI>(g, sm.red, pos, ext_out, zg[0], zg[1]);
   }
@@ -206,9 +214,9 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
                                           cplx<R>* msgB, BpStatus* st, cplx<double>* ext_out, double* rho_out,
                                           double* bloch_out, int start) {
   static_assert(D >= 2 && D <= 5 && CHI >= 2 && CHI <= 8, "shape");
-  static_assert(D <= 4 || MODE == kModeBP, "degree 5: BP sweeps only");
+  static_assert(D <= 4 || MODE != kModeDensity, "degree 5: BP sweeps and extended messages");
   using M = Sm<D, CHI>;
-  __shared__ __align__(16) SmallSmem<D, CHI, C> sm;
+  __shared__ __align__(16) SmallSmem<D, CHI, C, SmallGC<MODE, C>> sm;
   const int n = cv.n;
   const int l = threadIdx.x & 63;
   const cplx<R>* min = msgA;
@@ -643,9 +651,15 @@ inline bool launch_small(ClassView cv, int chi, const cplx<R>* T, cplx<R>* mA, c
                          cplx<double>* ext, double* rho, double* bloch, hipStream_t s, int policy = 0) {
   if (cv.n == 0) return false;
   if (cv.d == 5) {
-    if constexpr (MODE == kModeBP && sizeof(C) == sizeof(float) && sizeof(R) == sizeof(float)) {
-      if (small_bp_shape<R>(cv.d, chi))
-        return launch_small_d<5, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
+    // FP32 storage, chi <= 4: BP in FP32, extended messages from FP32 mode products with FP64 Grams
+    // (three 16 KB buffers at chi = 4; FP64 buffers would need 96 KB per one-wave block)
+    if constexpr (sizeof(R) == sizeof(float)) {
+      if (small_bp_shape<R>(cv.d, chi)) {
+        if constexpr (MODE == kModeBP && sizeof(C) == sizeof(float))
+          return launch_small_d<5, MODE, R, C>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
+        if constexpr (MODE == kModeExt)
+          return launch_small_d<5, MODE, R, float>(chi, cv, T, mA, mB, st, ext, rho, bloch, s, policy);
+      }
     }
     return false;
   }

@@ -25,9 +25,7 @@ CASES = {
     "grid5x7_chi8": lambda: instances.grid_gaussian(5, 7, seed=22, field_sigma=0.1),
 }
 # The heavy-hex patch has degree-1 nodes, whose sweep runs the generic kernel: its BP stays eager.
-# The grid's degree-4 nodes have specialised kernels only up to chi = 4, so whether it replays
-# graphs depends on how long its bond dimension stays there (None: not asserted).
-USES_GRAPHS = {"rr60_chi8": True, "heavyhex_chi8": False, "rr40_chi4": True, "grid5x7_chi8": None}
+USES_GRAPHS = {"rr60_chi8": True, "heavyhex_chi8": False, "rr40_chi4": True, "grid5x7_chi8": True}
 
 
 def _ctx(name, steps=40):
