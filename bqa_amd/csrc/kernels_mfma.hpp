@@ -935,12 +935,14 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_bp64(ClassView cv, const cplx<
 // four waves per CU (100k qubits: 1.21 ms per sweep against 1.35 ms for the
 // nine-contraction schedule with two waves per node; 1.28 ms without the
 // register prefetch below).
+// Two sets of Gram partials alternate, so a Gram may start while the first wave still
+// finishes the message of the previous one (64 KB per block, two blocks per CU).
 template <int W>
 struct Mf8Bp64S3Smem {
   double T[2 * Mf8<3>::S];
   double Y0[2 * Mf8<3>::S];
   double Y1[2 * Mf8<3>::S];
-  double Gp[W][256];
+  double Gp[2][W][256];
 };
 
 template <int W>
@@ -1025,18 +1027,17 @@ __global__ __launch_bounds__(64 * W) void k_mfma8_bp64_3p(ClassView cv, const cp
     mp8dw<D, W, 0, false>(sm.T, sm.Y0, bm[0], bm[0], zl);   // Y0 = T x0 m0
     mp8dw<D, W, 1, false>(sm.T, sm.Y1, bm[1], bm[1], zl);   // Y1 = T x1 m1
     node_sync<W>();
-    gram8_bp64<D, W, 2>(sm.Y1, sm.Y0, sm.Gp, zl);           // out_2: conj(Y1) Y0
+    gram8_bp64<D, W, 2>(sm.Y1, sm.Y0, sm.Gp[0], zl);        // out_2: conj(Y1) Y0
     node_sync<W>();
-    if (lead) finish8<W, double>(sm.Gp, pos[2], old[2], mout, dnum, dden);
+    if (lead) finish8<W, double>(sm.Gp[0], pos[2], old[2], mout, dnum, dden);
     mp8dw<D, W, 2, false>(sm.T, sm.T, bm[2], bm[2], zl);    // T <- Y2 = T x2 m2, in place
     node_sync<W>();
-    gram8_bp64<D, W, 0>(sm.T, sm.Y1, sm.Gp, zl);            // out_0: conj(Y2) Y1
+    gram8_bp64<D, W, 0>(sm.T, sm.Y1, sm.Gp[1], zl);         // out_0: conj(Y2) Y1
     node_sync<W>();
-    if (lead) finish8<W, double>(sm.Gp, pos[0], old[0], mout, dnum, dden);
-    node_sync<W>();                                          // the partials are read before the next Gram
-    gram8_bp64<D, W, 1>(sm.T, sm.Y0, sm.Gp, zl);            // out_1: conj(Y2) Y0
+    if (lead) finish8<W, double>(sm.Gp[1], pos[0], old[0], mout, dnum, dden);
+    gram8_bp64<D, W, 1>(sm.T, sm.Y0, sm.Gp[0], zl);         // out_1: conj(Y2) Y0 (the other partials)
     node_sync<W>();
-    if (lead) finish8<W, double>(sm.Gp, pos[1], old[1], mout, dnum, dden);
+    if (lead) finish8<W, double>(sm.Gp[0], pos[1], old[1], mout, dnum, dden);
     // the next node's tensor load overwrites T, which the last Gram read
     node_sync<W>();
   }
