@@ -256,6 +256,18 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One butterfly step of the pivot search: the larger diagonal of this lane and lane ^ M, ties to
+// the lower column (-1: none).
+template <int M>
+__device__ __forceinline__ void pivot_step(double& bv, int& bc) {
+  const double ov = xor_lane<M>(bv);
+  const int oc = xor_lane32<M>(bc);
+  if (ov > bv || (ov == bv && oc >= 0 && (bc < 0 || oc < bc))) {
+    bv = ov;
+    bc = oc;
+  }
+}
+
 // Pivoted Cholesky M = L L^H in place: each column ends up holding one column of
 // L (in pivot order) or zero beyond the numerical rank.  The scaled pivot
 // column goes through the edge's LDS slot `lcol` (N complex).  dgt / dgb are the
@@ -274,12 +286,12 @@ __device__ void hcholesky(HCol<N / 2>& top, HCol<N / 2>& bot, int k, int h, bool
     int bc = -1;
     if (!dt && dgt > bv) { bv = dgt; bc = ct; }
     if (!db && dgb > bv) { bv = dgb; bc = cb; }
-#pragma unroll
-    for (int o = 1; o < P2; o <<= 1) {
-      const double ov = __shfl_xor(bv, o, 64);
-      const int oc = __shfl_xor(bc, o, 64);
-      if (ov > bv || (ov == bv && oc >= 0 && (bc < 0 || oc < bc))) { bv = ov; bc = oc; }
-    }
+    // butterfly over the column pairs by DPP lane xors (a step's latency is a few cycles, where a
+    // shuffle through the LDS crossbar waits for its round trip); the same order as a shuffle loop
+    if constexpr (P2 > 1) pivot_step<1>(bv, bc);
+    if constexpr (P2 > 2) pivot_step<2>(bv, bc);
+    if constexpr (P2 > 4) pivot_step<4>(bv, bc);
+    if constexpr (P2 > 8) pivot_step<8>(bv, bc);
     if (step == 0) d0 = bv;
     if (!(bv > 1e-20 * d0) || bc < 0) {  // numerical rank reached: zero the rest
       if (!dt) {
