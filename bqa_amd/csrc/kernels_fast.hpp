@@ -31,7 +31,7 @@ int bp_variant_count(int d, int chi);
 // Shapes whose BP sweep has a specialised kernel (fast_bp_launch returns true for them).
 template <typename R>
 inline bool bp_fast_shape(int d, int chi) {
-  if ((chi == 8 || (sizeof(R) == sizeof(float) && chi == 16)) && (d == 2 || d == 3)) return true;
+  if ((chi == 8 || chi == 16) && (d == 2 || d == 3)) return true;
   if (chi == 8 && d == 4) return true;
   return small_bp_shape<R>(d, chi);
 }
@@ -234,14 +234,32 @@ inline const char* bp_variant_name<double>(int d, int chi, int v) {
                                               "k_small_npw<2> (2 nodes per wave), resident",
                                               "k_small_npw<4> (4 nodes per wave), resident"};
   if (chi == 8 && d == 4) return v == 0 ? "k_bp_d4c8<256 threads>" : "k_bp_d4c8<512 threads>";
+  if (chi == 16 && (d == 2 || d == 3)) return d == 3 ? "k_mfma16_bp64<3,8>" : "k_mfma16_bp64<2,1>";
   if (chi == 8 && d == 3) return b3[v % (kGridPolicies + 1)];
   if (chi == 8 && d == 2) return b2[v % kGridPolicies];
   return small_shape(d, chi) ? sm[v % (kGridPolicies + 2)] : "generic";
 }
 
+// FP64, chi = 16, degree 2 or 3: k_mfma16_bp64 (eight waves per degree-3 node, one node per CU;
+// one wave per degree-2 node).
+template <int D, int W>
+inline void launch_mfma16_bp64(ClassView cv, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
+                               BpStatus* st, hipStream_t s) {
+  constexpr int LDS = Mf16D<D, W>::BYTES;
+  const int grid = node_grid_lds<k_mfma16_bp64<D, W>, 64 * W, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_mfma16_bp64<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, mA, mB, st);
+}
+
 template <>
 inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
                                    BpStatus* st, hipStream_t s, int v) {
+  if (chi == 16 && cv.n > 0 && (cv.d == 2 || cv.d == 3)) {
+    if (cv.d == 3)
+      launch_mfma16_bp64<3, 8>(cv, T, mA, mB, st, s);
+    else
+      launch_mfma16_bp64<2, 1>(cv, T, mA, mB, st, s);
+    return true;
+  }
   if (v >= kGridPolicies && npw_shape(cv.d, chi))
     return v == kGridPolicies ? launch_small_npw<2, double>(cv, chi, T, mA, mB, st, s)
                               : launch_small_npw<4, double>(cv, chi, T, mA, mB, st, s);

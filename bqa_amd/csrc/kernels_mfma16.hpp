@@ -423,6 +423,225 @@ __global__ __launch_bounds__(64 * W) void k_mfma16_bp3h(ClassView cv, const cplx
 }
 
 // ---------------------------------------------------------------------------
+// BP sweep of an FP64 job at chi = 16, degree 2 or 3 (k_mfma16_bp64), on v_mfma_f64_16x16x4f64:
+// the reference's nine-contraction schedule with the FP32 kernel's tiling, complex128 elements
+// (double2) at the same swizzled positions.  A degree-3 node (128 KB) runs one physical half at
+// a time through a 64 KB tensor buffer and a 64 KB work buffer, the Grams (which sum over p)
+// adding up in FP64 registers; a degree-2 node (8 KB) runs whole.  FP64 MFMA result register v of
+// lane l holds D[l / 16 + 4 v][l % 16].
+
+// B operand of message m for a mode product (as msg_operand16, in FP64).
+__device__ __forceinline__ void msg_operand16d(const cplx<double>* __restrict__ m, double (&bm)[8][2]) {
+  const int l = threadIdx.x & 63, a = l & 15, kq = l >> 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const cplx<double> v = m[a * 16 + 4 * q + kq];
+    bm[q][0] = v.x;
+    bm[q][1] = v.y;
+    bm[q + 4][0] = -v.y;
+    bm[q + 4][1] = v.x;
+  }
+}
+
+// Y = X x_J m over this wave's tiles of a buffer holding H = 1 or 2 halves' worth of fibres (the
+// first 1/H of them); X may equal Y (a wave reads its tiles whole before writing them).
+template <int D, int W, int J, int H>
+__device__ __forceinline__ void mp16d(const double2* X, double2* Y, const cplx<double>* __restrict__ m) {
+  double bm[8][2];
+  msg_operand16d(m, bm);
+  constexpr int st = cpow16(D - 1 - J);
+  constexpr int TPW = Mf16<D>::TILES / H / W;
+  static_assert(TPW >= 1 && Mf16<D>::TILES % (H * W) == 0, "tiles split evenly over the waves");
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, i16 = l & 15, kq = l >> 4, w = tid >> 6;
+  double a[TPW][8];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int f0 = (w * TPW + t) * 16;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double2 v = X[sw16<D>(fbase16<D, J>(f0 + i16) + (4 * q + kq) * st)];
+      a[t][q] = v.x;
+      a[t][q + 4] = v.y;
+    }
+  }
+  d4v acc[TPW][2];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t][0] = acc[t][1] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      acc[t][0] = mfma4d(a[t][s], bm[s][0], acc[t][0]);
+      acc[t][1] = mfma4d(a[t][s], bm[s][1], acc[t][1]);
+    }
+  // D[kq + 4 v][i16]: fibre f0 + kq + 4 v, element i16 along J
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int f0 = (w * TPW + t) * 16;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      Y[sw16<D>(fbase16<D, J>(f0 + kq + 4 * v) + i16 * st)] = make_double2(acc[t][0][v], acc[t][1][v]);
+  }
+}
+
+// This wave's k-steps of out[c][e] = sum_f conj(X[f][c]) Y[f][e] over the buffer's fibres along
+// bond K, added into gre/gim[v] = entry (c = l / 16 + 4 v, e = l % 16).
+template <int D, int W, int K, int H>
+__device__ __forceinline__ void gram16d(const double2* X, const double2* Y, double (&gre)[4], double (&gim)[4]) {
+  constexpr int st = cpow16(D - 1 - K);
+  constexpr int KPW = Mf16<D>::KS / H / W;
+  static_assert(KPW >= 2 && KPW % 2 == 0, "two chains per wave");
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, i16 = l & 15, kq = l >> 4, w = tid >> 6;
+  d4v gr[2], gi[2];
+  gr[0] = gr[1] = gi[0] = gi[1] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int s = 0; s < KPW; ++s) {
+    const int idx = sw16<D>(fbase16<D, K>(4 * (w * KPW + s) + kq) + i16 * st);
+    const double2 x = X[idx], y = Y[idx];
+    const int c = s & 1;
+    gr[c] = mfma4d(x.x, y.x, gr[c]);
+    gr[c] = mfma4d(x.y, y.y, gr[c]);
+    gi[c] = mfma4d(x.x, y.y, gi[c]);
+    gi[c] = mfma4d(-x.y, y.x, gi[c]);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    gre[v] += gr[0][v] + gr[1][v];
+    gim[v] += gi[0][v] + gi[1][v];
+  }
+}
+
+template <int D, int W>
+struct Mf16D {
+  static constexpr int H = D == 3 ? 2 : 1;                   // physical halves run one at a time
+  static constexpr int SB = Mf16<D>::S / H;                  // complex elements per buffer
+  static constexpr int FIN = W * 256 * 16 + 256 * 16;        // partials + summed Gram over the work buffer
+  static constexpr int BYTES = 2 * SB * 16 + (FIN > SB * 16 ? FIN - SB * 16 : 0);
+};
+
+template <int D, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma16_bp64(ClassView cv, const cplx<double>* __restrict__ T,
+                                                        cplx<double>* msgA, cplx<double>* msgB, BpStatus* st) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  using L = Mf16D<D, W>;
+  constexpr int S = Mf16<D>::S, H = L::H, SB = L::SB, NT = 64 * W;
+  static_assert(SB % NT == 0, "buffer loads in whole rounds");
+  extern __shared__ __align__(16) unsigned char smem16[];
+  double2* Ts = reinterpret_cast<double2*>(smem16);
+  double2* Wb = Ts + SB;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const int n = cv.n;
+  const cplx<double>* min = cur ? msgB : msgA;
+  cplx<double>* mout = cur ? msgA : msgB;
+  double dnum = 0.0, dden = 0.0;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, kq = l >> 4, e16 = l & 15;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<double>* m[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) m[j] = min + (size_t)cv.in_pos[j * n + i] * 256;
+    double gr[D][4], gi[D][4];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) gr[k][v] = gi[k][v] = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < H; ++p) {
+      __syncthreads();  // the previous half's (or node's) readers of the buffers are done
+      const cplx<double>* Tg = T + (size_t)i * S + (size_t)p * SB;
+#pragma unroll 4
+      for (int k = 0; k < SB / NT; ++k) {
+        const int x = threadIdx.x + NT * k;
+        const cplx<double> v = Tg[x];
+        Ts[sw16<D>(x)] = make_double2(v.x, v.y);
+      }
+      __syncthreads();
+      if constexpr (D == 2) {
+        mp16d<D, W, 1, H>(Ts, Wb, m[1]);                       // W = T x1 m1
+        __syncthreads();
+        gram16d<D, W, 0, H>(Ts, Wb, gr[0], gi[0]);
+        __syncthreads();
+        mp16d<D, W, 0, H>(Ts, Wb, m[0]);                       // W = T x0 m0
+        __syncthreads();
+        gram16d<D, W, 1, H>(Ts, Wb, gr[1], gi[1]);
+      } else {
+        mp16d<D, W, 2, H>(Ts, Wb, m[2]);                       // W = T x2 m2
+        __syncthreads();
+        mp16d<D, W, 1, H>(Wb, Wb, m[1]);                       // W = W x1 m1
+        __syncthreads();
+        gram16d<D, W, 0, H>(Ts, Wb, gr[0], gi[0]);
+        __syncthreads();
+        mp16d<D, W, 2, H>(Ts, Wb, m[2]);
+        __syncthreads();
+        mp16d<D, W, 0, H>(Wb, Wb, m[0]);                       // W = T x2 m2 x0 m0
+        __syncthreads();
+        gram16d<D, W, 1, H>(Ts, Wb, gr[1], gi[1]);
+        __syncthreads();
+        mp16d<D, W, 0, H>(Ts, Wb, m[0]);
+        __syncthreads();
+        mp16d<D, W, 1, H>(Wb, Wb, m[1]);                       // W = T x0 m0 x1 m1
+        __syncthreads();
+        gram16d<D, W, 2, H>(Ts, Wb, gr[2], gi[2]);
+      }
+    }
+    __syncthreads();  // every wave has read the work buffer
+    double2* part = Wb;
+    double2* Sg = part + W * 256;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part[w * 256 + (kq + 4 * v) * 16 + e16] = make_double2(gr[k][v], gi[k][v]);
+      __syncthreads();
+      for (int o = threadIdx.x; o < 256; o += NT) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+          re += part[q * 256 + o].x;
+          im += part[q * 256 + o].y;
+        }
+        Sg[o] = make_double2(re, im);
+      }
+      __syncthreads();
+      double tr = 0.0;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) tr += Sg[c * 17].x;
+      const double inv = 1.0 / tr;
+      const long pos = cv.out_pos[k * n + i];
+      for (int o = threadIdx.x; o < 256; o += NT) {
+        const int c = o >> 4, e = o & 15;
+        const double2 a = Sg[o], b = Sg[e * 16 + c];
+        const cplx<double> v = mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv);
+        const cplx<double> od = min[pos * 256 + o];
+        dnum = fmax(dnum, abs2(v - od));
+        dden = fmax(dden, abs2(v + od));
+        mout[pos * 256 + o] = v;
+      }
+      __syncthreads();
+    }
+  }
+  __shared__ double red[2][W];
+  for (int o = 32; o > 0; o >>= 1) {
+    dnum = fmax(dnum, __shfl_xor(dnum, o, 64));
+    dden = fmax(dden, __shfl_xor(dden, o, 64));
+  }
+  if (l == 0) {
+    red[0][w] = dnum;
+    red[1][w] = dden;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+    for (int k = 0; k < W; ++k) {
+      x = fmax(x, red[0][k]);
+      y = fmax(y, red[1][k]);
+    }
+    bp_end(st, cv.sweep, sqrt(x), sqrt(y));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Extended messages of an FP32 job at chi = 16, degree 2 or 3 (state.py:127-139,
 // backends.py:391-408): the mode products of k_mfma16_bp in FP32 MFMA, then for
 // every bond K the two physical-index-resolved Grams
