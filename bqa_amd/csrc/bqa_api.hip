@@ -880,7 +880,7 @@ void ext_and_canon(bqa_job* j, double ztime, double pinv_eps, bool sync = true, 
   for (auto& c : j->cls)
     need_generic |= c.d != 0 && c.n > 0 &&
                     (j->generic(kGenExt) || !((chi == 8 && c.d >= 2 && c.d <= 4) || small_shape(c.d, chi) ||
-                                              (sizeof(R) == sizeof(float) && chi == 16 && (c.d == 2 || c.d == 3)) ||
+                                              (chi == 16 && (c.d == 2 || c.d == 3)) ||
                                               (sizeof(R) == sizeof(float) && c.d == 5 && small_bp_shape<R>(5, chi))));
   if (sizeof(R) == sizeof(float) && need_generic) {
     const long cnt = (long)j->M * chi * chi;

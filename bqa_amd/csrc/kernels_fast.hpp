@@ -65,21 +65,34 @@ inline void launch_mfma16_ext(ClassView cv, const cplx<float>* T, const cplx<flo
   hipLaunchKernelGGL((k_mfma16_ext<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, msgs, ext);
 }
 
-// Extended messages in FP64 from R storage: MFMA for chi = 8 (degree 2, 3) and for FP32
-// jobs at chi = 16 (degree 2, 3), the degree-4 chi = 8 kernel, k_small for the other served
-// shapes; false leaves the generic kernel.  The ZZ-split gates of the step come from cv.zz.
+// Extended messages in FP64 from R storage: MFMA for chi = 8 and chi = 16 (degree 2, 3), the
+// degree-4 chi = 8 kernel, k_small for the other served shapes; false leaves the generic kernel.  The ZZ-split gates of the step come from cv.zz.
+// FP64, chi = 16, degree 2 or 3: k_mfma16_ext64, one physical half at a time.
+template <int D, int W>
+inline void launch_mfma16_ext64(ClassView cv, const cplx<double>* T, const cplx<double>* msgs, cplx<double>* ext,
+                                hipStream_t s) {
+  constexpr int LDS = Mf16DExt<D, W>::BYTES;
+  const int grid = node_grid_lds<k_mfma16_ext64<D, W>, 64 * W, LDS>(cv.n);
+  hipLaunchKernelGGL((k_mfma16_ext64<D, W>), dim3(grid), dim3(64 * W), LDS, s, cv, T, msgs, ext);
+}
+
 template <typename R>
 bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext,
                      hipStream_t s) {
   if (cv.n == 0 || cv.zz == nullptr) return false;
-  if constexpr (sizeof(R) == sizeof(float)) {
-    if (chi == 16 && (cv.d == 2 || cv.d == 3)) {
+  if (chi == 16 && (cv.d == 2 || cv.d == 3)) {
+    if constexpr (sizeof(R) == sizeof(float)) {
       if (cv.d == 3)
         launch_mfma16_ext<3, 8>(cv, T, msgs, ext, s);
       else
         launch_mfma16_ext<2, 1>(cv, T, msgs, ext, s);
-      return true;
+    } else {
+      if (cv.d == 3)
+        launch_mfma16_ext64<3, 8>(cv, T, msgs, ext, s);
+      else
+        launch_mfma16_ext64<2, 1>(cv, T, msgs, ext, s);
     }
+    return true;
   }
   if (chi == 8 && cv.d == 4) {
     launch_ext_d4c8<R, 512>(cv, T, msgs, ext, s);

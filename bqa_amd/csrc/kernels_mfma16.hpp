@@ -521,6 +521,40 @@ struct Mf16D {
   static constexpr int BYTES = 2 * SB * 16 + (FIN > SB * 16 ? FIN - SB * 16 : 0);
 };
 
+// One buffer's share of the nine-contraction schedule (the whole node, or one physical half),
+// adding its three (two) Grams into gr/gi[k]; on return the last Gram may still be reading Wb.
+template <int D, int W, int H>
+__device__ __forceinline__ void bp16d_schedule(const double2* Ts, double2* Wb, const cplx<double>* const (&m)[D],
+                                               double (&gr)[D][4], double (&gi)[D][4]) {
+  if constexpr (D == 2) {
+    mp16d<D, W, 1, H>(Ts, Wb, m[1]);                       // W = T x1 m1
+    __syncthreads();
+    gram16d<D, W, 0, H>(Ts, Wb, gr[0], gi[0]);
+    __syncthreads();
+    mp16d<D, W, 0, H>(Ts, Wb, m[0]);                       // W = T x0 m0
+    __syncthreads();
+    gram16d<D, W, 1, H>(Ts, Wb, gr[1], gi[1]);
+  } else {
+    mp16d<D, W, 2, H>(Ts, Wb, m[2]);                       // W = T x2 m2
+    __syncthreads();
+    mp16d<D, W, 1, H>(Wb, Wb, m[1]);                       // W = W x1 m1
+    __syncthreads();
+    gram16d<D, W, 0, H>(Ts, Wb, gr[0], gi[0]);
+    __syncthreads();
+    mp16d<D, W, 2, H>(Ts, Wb, m[2]);
+    __syncthreads();
+    mp16d<D, W, 0, H>(Wb, Wb, m[0]);                       // W = T x2 m2 x0 m0
+    __syncthreads();
+    gram16d<D, W, 1, H>(Ts, Wb, gr[1], gi[1]);
+    __syncthreads();
+    mp16d<D, W, 0, H>(Ts, Wb, m[0]);
+    __syncthreads();
+    mp16d<D, W, 1, H>(Wb, Wb, m[1]);                       // W = T x0 m0 x1 m1
+    __syncthreads();
+    gram16d<D, W, 2, H>(Ts, Wb, gr[2], gi[2]);
+  }
+}
+
 template <int D, int W>
 __global__ __launch_bounds__(64 * W) void k_mfma16_bp64(ClassView cv, const cplx<double>* __restrict__ T,
                                                         cplx<double>* msgA, cplx<double>* msgB, BpStatus* st) {
@@ -558,33 +592,7 @@ __global__ __launch_bounds__(64 * W) void k_mfma16_bp64(ClassView cv, const cplx
         Ts[sw16<D>(x)] = make_double2(v.x, v.y);
       }
       __syncthreads();
-      if constexpr (D == 2) {
-        mp16d<D, W, 1, H>(Ts, Wb, m[1]);                       // W = T x1 m1
-        __syncthreads();
-        gram16d<D, W, 0, H>(Ts, Wb, gr[0], gi[0]);
-        __syncthreads();
-        mp16d<D, W, 0, H>(Ts, Wb, m[0]);                       // W = T x0 m0
-        __syncthreads();
-        gram16d<D, W, 1, H>(Ts, Wb, gr[1], gi[1]);
-      } else {
-        mp16d<D, W, 2, H>(Ts, Wb, m[2]);                       // W = T x2 m2
-        __syncthreads();
-        mp16d<D, W, 1, H>(Wb, Wb, m[1]);                       // W = W x1 m1
-        __syncthreads();
-        gram16d<D, W, 0, H>(Ts, Wb, gr[0], gi[0]);
-        __syncthreads();
-        mp16d<D, W, 2, H>(Ts, Wb, m[2]);
-        __syncthreads();
-        mp16d<D, W, 0, H>(Wb, Wb, m[0]);                       // W = T x2 m2 x0 m0
-        __syncthreads();
-        gram16d<D, W, 1, H>(Ts, Wb, gr[1], gi[1]);
-        __syncthreads();
-        mp16d<D, W, 0, H>(Ts, Wb, m[0]);
-        __syncthreads();
-        mp16d<D, W, 1, H>(Wb, Wb, m[1]);                       // W = T x0 m0 x1 m1
-        __syncthreads();
-        gram16d<D, W, 2, H>(Ts, Wb, gr[2], gi[2]);
-      }
+      bp16d_schedule<D, W, H>(Ts, Wb, m, gr, gi);
     }
     __syncthreads();  // every wave has read the work buffer
     double2* part = Wb;
@@ -638,6 +646,93 @@ __global__ __launch_bounds__(64 * W) void k_mfma16_bp64(ClassView cv, const cplx
       y = fmax(y, red[1][k]);
     }
     bp_end(st, cv.sweep, sqrt(x), sqrt(y));
+  }
+}
+
+// Extended messages of an FP64 job at chi = 16, degree 2 or 3 (k_mfma16_ext64): the schedule of
+// k_mfma16_bp64 run one physical half at a time (degree 2 too), so each half's Grams are the
+// p-resolved Grams G_p; they are summed over the waves into LDS after each half, and every bond's
+// 32 x 32 message is assembled from G_0, G_1 and its ZZ-split gates as in k_mfma16_ext.
+template <int D, int W>
+struct Mf16DExt {
+  static constexpr int SB = Mf16<D>::S / 2;                  // one physical half
+  static constexpr int PART = W * 256 * 16;                  // per-wave partials over the work buffer
+  static constexpr int BYTES = 2 * SB * 16 + (PART > SB * 16 ? PART - SB * 16 : 0) + 2 * D * 256 * 16;
+};
+
+template <int D, int W>
+__global__ __launch_bounds__(64 * W) void k_mfma16_ext64(ClassView cv, const cplx<double>* __restrict__ T,
+                                                         const cplx<double>* __restrict__ msgs,
+                                                         cplx<double>* __restrict__ ext_out) {
+  static_assert(D == 2 || D == 3, "degree 2 or 3");
+  using L = Mf16DExt<D, W>;
+  constexpr int S = Mf16<D>::S, H = 2, SB = L::SB, NT = 64 * W;
+  static_assert(SB % NT == 0, "half loads in whole rounds");
+  extern __shared__ __align__(16) unsigned char smem16[];
+  double2* Ts = reinterpret_cast<double2*>(smem16);
+  double2* Wb = Ts + SB;
+  double2* Gs = reinterpret_cast<double2*>(smem16 + L::BYTES - 2 * D * 256 * 16);  // [p][k][256]
+  const int n = cv.n;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, kq = l >> 4, e16 = l & 15;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const cplx<double>* m[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) m[j] = msgs + (size_t)cv.in_pos[j * n + i] * 256;
+#pragma unroll 1
+    for (int p = 0; p < H; ++p) {
+      __syncthreads();  // the previous half's (or node's) readers of the buffers are done
+      const cplx<double>* Tg = T + (size_t)i * S + (size_t)p * SB;
+#pragma unroll 4
+      for (int k = 0; k < SB / NT; ++k) {
+        const int x = threadIdx.x + NT * k;
+        const cplx<double> v = Tg[x];
+        Ts[sw16<D>(x)] = make_double2(v.x, v.y);
+      }
+      double gr[D][4], gi[D][4];
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) gr[k][v] = gi[k][v] = 0.0;
+      __syncthreads();
+      bp16d_schedule<D, W, H>(Ts, Wb, m, gr, gi);
+      __syncthreads();  // every wave has read the work buffer
+      double2* part = Wb;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part[w * 256 + (kq + 4 * v) * 16 + e16] = make_double2(gr[k][v], gi[k][v]);
+        __syncthreads();
+        for (int o = threadIdx.x; o < 256; o += NT) {
+          double re = 0.0, im = 0.0;
+#pragma unroll
+          for (int q = 0; q < W; ++q) {
+            re += part[q * 256 + o].x;
+            im += part[q * 256 + o].y;
+          }
+          Gs[(p * D + k) * 256 + o] = make_double2(re, im);
+        }
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double2* G0 = Gs + k * 256;
+      const double2* G1 = Gs + (D + k) * 256;
+      cplx<double> tr = mk(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) tr = tr + mk(G0[c * 17].x + G1[c * 17].x, G0[c * 17].y + G1[c * 17].y);
+      const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
+      const cplx<double> g0 = zg[0], g1 = zg[1];
+      const cplx<double> inv = cinv(tr * mk(abs2(g0) + abs2(g1), 0.0));
+      cplx<double>* out = ext_out + (size_t)cv.out_pos[k * n + i] * 1024;
+      for (int o = threadIdx.x; o < 1024; o += NT) {
+        const int row = o >> 5, col = o & 31;
+        const int s_ = row >> 4, c = row & 15, s2 = col >> 4, e = col & 15;
+        const double2 a = G0[c * 16 + e], b = G1[c * 16 + e];
+        const cplx<double> gsum = ((s_ + s2) & 1) ? mk(a.x - b.x, a.y - b.y) : mk(a.x + b.x, a.y + b.y);
+        out[o] = conj(s_ ? g1 : g0) * (s2 ? g1 : g0) * gsum * inv;
+      }
+    }
   }
 }
 
