@@ -481,4 +481,203 @@ inline void launch_bp_d4c8m(ClassView cv, const cplx<float>* T, cplx<float>* mA,
   hipLaunchKernelGGL(k_bp_d4c8m, dim3(grid), dim3(256), LDS, s, cv, T, mA, mB, st);
 }
 
+// ---------------------------------------------------------------------------
+// The same sweep for FP64 jobs on v_mfma_f64_16x16x4f64 (k_bp_d4c8m64<NW>; NW waves per node, the
+// schedule, LDS layout and swizzle of k_bp_d4c8m).  The A and B operand lane layouts are those of
+// the FP32 instruction; result register v of lane l holds D[l / 16 + 4 v][l % 16].  The two FP64
+// halves take 128 KB of LDS, so one block runs per CU; NW = 8 gives two waves per SIMD.
+
+// In-place mode product along bond J with message m (64 complex entries, m[a][b] at a * 8 + b).
+template <int J, int NW>
+__device__ __forceinline__ void d4m64_mp(cplx<double>* X, const cplx<double>* m) {
+  constexpr int st = J == 0 ? 512 : J == 1 ? 64 : J == 2 ? 8 : 1;
+  constexpr int TPW = 32 / NW;  // 32 tiles of 16 fibres per half
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, w = tid >> 6, i16 = l & 15, kq = l >> 4;
+  const int a = i16 & 7, co = i16 >> 3;
+  double bm[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * s + kq, ci = k >> 3, b = k & 7;
+    const cplx<double> mv = m[a * 8 + b];
+    bm[s] = ci == co ? mv.x : (ci ? -mv.y : mv.y);
+  }
+  double xa[TPW][4];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int f = (w * TPW + t) * 16 + i16;
+    const int base = (f / st) * 8 * st + f % st;
+    const cplx<double> x0 = X[d4_sw(base + kq * st)], x1 = X[d4_sw(base + (kq + 4) * st)];
+    xa[t][0] = x0.x;
+    xa[t][1] = x1.x;
+    xa[t][2] = x0.y;
+    xa[t][3] = x1.y;
+  }
+  d4v acc[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) acc[t] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = mfma4d(xa[t][s], bm[s], acc[t]);
+  double* Xd = reinterpret_cast<double*>(X);
+#pragma unroll
+  for (int t = 0; t < TPW; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int f = (w * TPW + t) * 16 + kq + 4 * v;
+      const int base = (f / st) * 8 * st + f % st;
+      Xd[2 * d4_sw(base + a * st) + co] = acc[t][v];
+    }
+}
+
+// This wave's k-steps of the Gram tile along bond K, added into g[v] = D[l / 16 + 4 v][l % 16]
+// (D[(comp, c)][(comp', e)] = sum over fibres of A[f][c].comp B[f][e].comp').
+template <int K, int NW>
+__device__ __forceinline__ void d4m64_gram(const cplx<double>* A, const cplx<double>* B, double (&g)[4]) {
+  constexpr int st = K == 0 ? 512 : K == 1 ? 64 : K == 2 ? 8 : 1;
+  constexpr int KPW = 128 / NW;  // 128 k-steps of four fibres per half
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, w = tid >> 6, x = l & 7, comp = (l >> 3) & 1, kq = l >> 4;
+  const double* Ad = reinterpret_cast<const double*>(A);
+  const double* Bd = reinterpret_cast<const double*>(B);
+  d4v acc[4] = {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0},
+                d4v{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll 4
+  for (int s = 0; s < KPW; ++s) {
+    const int f = 4 * (w * KPW + s) + kq;
+    const int idx = 2 * d4_sw((f / st) * 8 * st + f % st + x * st) + comp;
+    acc[s & 3] = mfma4d(Ad[idx], Bd[idx], acc[s & 3]);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) g[v] += (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+}
+
+// One physical half: d4_half's schedule on the FP64 MFMA products.
+template <int NW>
+__device__ __forceinline__ void d4m64_half(cplx<double>* A, cplx<double>* B, const cplx<double>* M,
+                                           const cplx<double>* __restrict__ Tp, double (&g)[4][4]) {
+  constexpr int H = D4Lds<double>::H, NT = 64 * NW;
+  auto load = [&](cplx<double>* X) {
+    const int t = opaque((int)threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < H / NT; ++q) X[d4_sw(t + NT * q)] = Tp[t + NT * q];
+  };
+  load(B);
+  d4_sync();
+  d4m64_mp<2, NW>(B, M + 2 * 64);
+  d4_sync();
+  d4m64_mp<3, NW>(B, M + 3 * 64);  // B = T x2 m2 x3 m3
+  load(A);
+  d4_sync();
+  d4m64_mp<1, NW>(A, M + 1 * 64);  // A = T x1 m1
+  d4_sync();
+  d4m64_gram<0, NW>(A, B, g[0]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4m64_mp<0, NW>(A, M + 0 * 64);  // A = T x0 m0
+  d4_sync();
+  d4m64_gram<1, NW>(A, B, g[1]);
+  d4_sync();
+  load(B);
+  d4_sync();
+  d4m64_mp<0, NW>(B, M + 0 * 64);
+  d4_sync();
+  d4m64_mp<1, NW>(B, M + 1 * 64);  // B = T x0 m0 x1 m1
+  load(A);
+  d4_sync();
+  d4m64_mp<3, NW>(A, M + 3 * 64);  // A = T x3 m3
+  d4_sync();
+  d4m64_gram<2, NW>(A, B, g[2]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4m64_mp<2, NW>(A, M + 2 * 64);  // A = T x2 m2
+  d4_sync();
+  d4m64_gram<3, NW>(A, B, g[3]);
+}
+
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_bp_d4c8m64(ClassView cv, const cplx<double>* __restrict__ T,
+                                                        cplx<double>* msgA, cplx<double>* msgB, BpStatus* st) {
+  static_assert(NW == 4 || NW == 8, "four or eight waves per node");
+  constexpr int H = D4Lds<double>::H;
+  extern __shared__ __align__(16) unsigned char d4_lds[];
+  cplx<double>* A = reinterpret_cast<cplx<double>*>(d4_lds);
+  cplx<double>* B = A + H;
+  cplx<double>* M = B + H;
+  const int cur = bp_begin(st, cv.sweep);
+  if (cur < 0) return;
+  const cplx<double>* min = cur ? msgB : msgA;
+  cplx<double>* mout = cur ? msgA : msgB;
+  const int n = cv.n;
+  const int t = threadIdx.x, o = t & 63;
+  double dnum = 0.0, dden = 0.0;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    d4_sync();  // the previous node's last readers of A, B and M are done
+    if (t < 256) M[(t >> 6) * 64 + o] = min[(size_t)cv.in_pos[(t >> 6) * n + i] * 64 + o];
+    double g[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g[k][v] = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      if (p) d4_sync();  // the last Gram of the first half has read B
+      d4m64_half<NW>(A, B, M, T + (size_t)i * 2 * H + (size_t)p * H, g);
+    }
+    // the waves' Gram tiles into the dead A buffer ([k][wave][256] doubles: 32 or 64 KB), then one
+    // wave finishes
+    d4_sync();
+    double* red = reinterpret_cast<double*>(A);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[(k * NW + (t >> 6)) * 256 + ((o >> 4) + 4 * v) * 16 + (o & 15)] = g[k][v];
+    d4_sync();
+    if (t < 64) {
+      const int c = o >> 3, e = o & 7;
+#pragma unroll 1
+      for (int k = 0; k < 4; ++k) {
+        auto dsum = [&](int r, int col) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NW; ++q) s += red[(k * NW + q) * 256 + r * 16 + col];
+          return s;
+        };
+        auto entry = [&](int cc, int ee) {
+          return mk(dsum(cc, ee) + dsum(8 + cc, 8 + ee), dsum(cc, 8 + ee) - dsum(8 + cc, ee));
+        };
+        double tr = 0.0;
+#pragma unroll
+        for (int d = 0; d < 8; ++d) tr += dsum(d, d) + dsum(8 + d, 8 + d);
+        const double inv = 1.0 / tr;
+        const cplx<double> a = entry(c, e), b = entry(e, c);
+        const cplx<double> v = mk(0.5 * (a.x + b.x) * inv, 0.5 * (a.y - b.y) * inv);
+        const size_t pos = (size_t)cv.out_pos[k * n + i];
+        const cplx<double> od = min[pos * 64 + o];
+        dnum = fmax(dnum, abs2(v - od));
+        dden = fmax(dden, abs2(v + od));
+        mout[pos * 64 + o] = v;
+      }
+    }
+  }
+  if (t < 64) {
+    for (int s = 32; s > 0; s >>= 1) {
+      dnum = fmax(dnum, __shfl_xor(dnum, s, 64));
+      dden = fmax(dden, __shfl_xor(dden, s, 64));
+    }
+    if (t == 0) bp_end(st, cv.sweep, sqrt(dnum), sqrt(dden));
+  }
+}
+
+template <int NW>
+inline void launch_bp_d4c8m64(ClassView cv, const cplx<double>* T, cplx<double>* mA, cplx<double>* mB,
+                              BpStatus* st, hipStream_t s) {
+  constexpr int LDS = D4Lds<double>::BYTES;
+  const int grid = node_grid_lds<k_bp_d4c8m64<NW>, 64 * NW, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_bp_d4c8m64<NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, mA, mB, st);
+}
+
 }  // namespace bqa

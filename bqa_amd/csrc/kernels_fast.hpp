@@ -224,12 +224,13 @@ inline bool fast_bp_launch<float>(ClassView cv, int chi, const cplx<float>* T, c
 //   chi = 8, d = 3: k_mfma8_bp64_3p<4> (six contractions per node over four waves) under the three
 //     grid policies, and k_mfma8_bp64_3p<2> (two waves per node);
 //   chi = 8, d = 2: k_mfma8_bp64<2, 1> (one wave per node) under the three grid policies;
-//   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node;
+//   chi = 8, d = 4: k_bp_d4c8 with 256 or 512 threads per node, and k_bp_d4c8m64 (FP64 MFMA) with
+//     four or eight waves per node;
 //   other served shapes: k_small under the three grid policies, and for degree 2, 3 with chi <= 4
 //   also k_small_npw with 2 and 4 nodes per wave.
 template <>
 inline int bp_variant_count<double>(int d, int chi) {
-  if (chi == 8 && d == 4) return 2;
+  if (chi == 8 && d == 4) return 4;
   if (chi == 8 && d == 3) return kGridPolicies + 1;
   if (chi == 8 && d == 2) return kGridPolicies;
   if (npw_shape(d, chi)) return kGridPolicies + 2;
@@ -246,7 +247,9 @@ inline const char* bp_variant_name<double>(int d, int chi, int v) {
   static const char* sm[kGridPolicies + 2] = {"k_small, 8192 blocks", "k_small, resident", "k_small, 2x resident",
                                               "k_small_npw<2> (2 nodes per wave), resident",
                                               "k_small_npw<4> (4 nodes per wave), resident"};
-  if (chi == 8 && d == 4) return v == 0 ? "k_bp_d4c8<256 threads>" : "k_bp_d4c8<512 threads>";
+  static const char* b4[4] = {"k_bp_d4c8<256 threads>", "k_bp_d4c8<512 threads>", "k_bp_d4c8m64<4 waves>",
+                              "k_bp_d4c8m64<8 waves>"};
+  if (chi == 8 && d == 4) return b4[v % 4];
   if (chi == 16 && (d == 2 || d == 3)) return d == 3 ? "k_mfma16_bp64<3,8>" : "k_mfma16_bp64<2,1>";
   if (chi == 8 && d == 3) return b3[v % (kGridPolicies + 1)];
   if (chi == 8 && d == 2) return b2[v % kGridPolicies];
@@ -279,6 +282,10 @@ inline bool fast_bp_launch<double>(ClassView cv, int chi, const cplx<double>* T,
   if (chi == 8 && cv.d == 4 && cv.n > 0) {
     if (v == 1)
       launch_bp_d4c8<double, 512>(cv, T, mA, mB, st, s);
+    else if (v == 2)
+      launch_bp_d4c8m64<4>(cv, T, mA, mB, st, s);
+    else if (v == 3)
+      launch_bp_d4c8m64<8>(cv, T, mA, mB, st, s);
     else
       launch_bp_d4c8<double, 256>(cv, T, mA, mB, st, s);
     return true;

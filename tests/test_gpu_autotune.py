@@ -36,8 +36,9 @@ CASES = {
     "heavyhex_chi16_fp32": (lambda: instances.heavy_hex_gaussian(6, 6, seed=44, field_sigma=0.5), 16, "single", 3,
                             2e-6),
     # degree 4 at chi = 8: k_bp_d4c8 with 256 or 512 threads per node (another fibre split of the Grams),
-    # and for FP32 also k_bp_d4c8m (FP32 MFMA, FP32 Gram chains added in FP64)
-    "grid24x24_chi8_fp64": (lambda: instances.grid_gaussian(24, 24, seed=47, field_sigma=0.1), 8, "double", 2,
+    # then for FP64 k_bp_d4c8m64 (FP64 MFMA) with four or eight waves per node, and for FP32
+    # k_bp_d4c8m (FP32 MFMA, FP32 Gram chains added in FP64)
+    "grid24x24_chi8_fp64": (lambda: instances.grid_gaussian(24, 24, seed=47, field_sigma=0.1), 8, "double", 4,
                             1e-13),
     "grid24x24_chi8_fp32": (lambda: instances.grid_gaussian(24, 24, seed=47, field_sigma=0.1), 8, "single", 3,
                             2e-6),
