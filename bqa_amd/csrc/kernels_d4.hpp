@@ -680,4 +680,88 @@ inline void launch_bp_d4c8m64(ClassView cv, const cplx<double>* T, cplx<double>*
   hipLaunchKernelGGL((k_bp_d4c8m64<NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, mA, mB, st);
 }
 
+// Extended messages of FP64 jobs at degree 4, chi = 8, on the FP64 MFMA products (k_ext_d4c8m64<NW>):
+// k_ext_d4c8's assembly from the per-half Grams, which here are the waves' 16 x 16 D tiles of
+// d4m64_gram, first reduced over the waves into complex 8 x 8 Grams [k][p] at the start of A.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_ext_d4c8m64(ClassView cv, const cplx<double>* __restrict__ T,
+                                                         const cplx<double>* __restrict__ msgs,
+                                                         cplx<double>* ext_out) {
+  static_assert(NW == 4 || NW == 8, "four or eight waves per node");
+  constexpr int H = D4Lds<double>::H, NT = 64 * NW;
+  extern __shared__ __align__(16) unsigned char d4_lds[];
+  cplx<double>* A = reinterpret_cast<cplx<double>*>(d4_lds);
+  cplx<double>* B = A + H;
+  cplx<double>* M = B + H;
+  const int n = cv.n;
+  const int t = threadIdx.x, o = t & 63;
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    d4_sync();  // the previous node's last readers of A, B and M are done
+    if (t < 256) M[(t >> 6) * 64 + o] = msgs[(size_t)cv.in_pos[(t >> 6) * n + i] * 64 + o];
+    double g[2][4][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) g[p][k][v] = 0.0;
+    d4m64_half<NW>(A, B, M, T + (size_t)i * 2 * H, g[0]);
+    d4_sync();
+    d4m64_half<NW>(A, B, M, T + (size_t)i * 2 * H + H, g[1]);
+    d4_sync();
+    // the D tiles [k][p][wave][256] (16 NW KB, in the dead A and B buffers, contiguous)
+    double* red = reinterpret_cast<double*>(A);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          red[((k * 2 + p) * NW + (t >> 6)) * 256 + ((o >> 4) + 4 * v) * 16 + (o & 15)] = g[p][k][v];
+    d4_sync();
+    // complex Gram entry (c, e) of bond k, half p: D[(0,c)][(0,e)] + D[(1,c)][(1,e)]
+    // + i (D[(0,c)][(1,e)] - D[(1,c)][(0,e)]), summed over the waves
+    constexpr int PER = 512 / NT;
+    cplx<double> ge[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int x = t + NT * u, kp = x >> 6, c = (x >> 3) & 7, e = x & 7;
+      auto dsum = [&](int r, int col) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) sum += red[(kp * NW + q) * 256 + r * 16 + col];
+        return sum;
+      };
+      ge[u] = mk(dsum(c, e) + dsum(8 + c, 8 + e), dsum(c, 8 + e) - dsum(8 + c, e));
+    }
+    d4_sync();
+    cplx<double>* gm = A;  // [k][p][64]
+#pragma unroll
+    for (int u = 0; u < PER; ++u) gm[t + NT * u] = ge[u];
+    d4_sync();
+    for (int x = t; x < 4 * 256; x += NT) {
+      const int k = x >> 8, e16 = x & 255, row = e16 >> 4, col = e16 & 15;
+      const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
+      cplx<double> tr = mk(0.0, 0.0);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) tr = tr + gm[(k * 2) * 64 + d * 9] + gm[(k * 2 + 1) * 64 + d * 9];
+      const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
+      const cplx<double> g0 = zg[0], g1 = zg[1];
+      const cplx<double> inv = cinv(tr * mk(abs2(g0) + abs2(g1), 0.0));
+      const cplx<double> a0 = gm[(k * 2) * 64 + c * 8 + e], a1 = gm[(k * 2 + 1) * 64 + c * 8 + e];
+      const cplx<double> gs = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
+      const cplx<double> ga = s ? g1 : g0, gb = s2 ? g1 : g0;
+      ext_out[(size_t)cv.out_pos[k * n + i] * 256 + e16] = conj(ga) * gb * gs * inv;
+    }
+  }
+}
+
+template <int NW>
+inline void launch_ext_d4c8m64(ClassView cv, const cplx<double>* T, const cplx<double>* msgs, cplx<double>* ext,
+                               hipStream_t s) {
+  constexpr int LDS = D4Lds<double>::BYTES;
+  const int grid = node_grid_lds<k_ext_d4c8m64<NW>, 64 * NW, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_ext_d4c8m64<NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, msgs, ext);
+}
+
 }  // namespace bqa
