@@ -403,6 +403,74 @@ __device__ __forceinline__ void d4m_half(cplx<float>* A, cplx<float>* B, const c
   d4m_gram<3>(A, B, g[3]);
 }
 
+// The Gram of d4m_gram for the extended messages of FP32 jobs: the FP32 operands widened on load
+// and multiplied and summed on v_mfma_f64_16x16x4f64 (exact products, FP64 sums), so
+// g[v] = D[l / 16 + 4 v][l % 16] (the FP64 result layout).
+template <int K>
+__device__ __forceinline__ void d4m_gram_w(const cplx<float>* A, const cplx<float>* B, double (&g)[4]) {
+  constexpr int st = K == 0 ? 512 : K == 1 ? 64 : K == 2 ? 8 : 1;
+  constexpr int KPW = 32;  // 128 k-steps of four fibres per half, four waves
+  const int tid = opaque((int)threadIdx.x);
+  const int l = tid & 63, w = tid >> 6, x = l & 7, comp = (l >> 3) & 1, kq = l >> 4;
+  const float* Af = reinterpret_cast<const float*>(A);
+  const float* Bf = reinterpret_cast<const float*>(B);
+  d4v acc[4] = {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0},
+                d4v{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll 4
+  for (int s = 0; s < KPW; ++s) {
+    const int f = 4 * (w * KPW + s) + kq;
+    const int idx = 2 * d4_sw((f / st) * 8 * st + f % st + x * st) + comp;
+    acc[s & 3] = mfma4d((double)Af[idx], (double)Bf[idx], acc[s & 3]);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) g[v] += (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
+}
+
+// One physical half for the extended messages of FP32 jobs: d4m_half's FP32 MFMA mode products with
+// the FP64 MFMA Grams of d4m_gram_w.
+__device__ __forceinline__ void d4m_ext_half(cplx<float>* A, cplx<float>* B, const cplx<float>* M,
+                                             const cplx<float>* __restrict__ Tp, double (&g)[4][4]) {
+  constexpr int H = D4Lds<float>::H, NT = 256;
+  auto load = [&](cplx<float>* X) {
+    const int t = opaque((int)threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < H / NT; ++q) X[d4_sw(t + NT * q)] = Tp[t + NT * q];
+  };
+  load(B);
+  d4_sync();
+  d4m_mp<2>(B, M + 2 * 64);
+  d4_sync();
+  d4m_mp<3>(B, M + 3 * 64);  // B = T x2 m2 x3 m3
+  load(A);
+  d4_sync();
+  d4m_mp<1>(A, M + 1 * 64);  // A = T x1 m1
+  d4_sync();
+  d4m_gram_w<0>(A, B, g[0]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4m_mp<0>(A, M + 0 * 64);  // A = T x0 m0
+  d4_sync();
+  d4m_gram_w<1>(A, B, g[1]);
+  d4_sync();
+  load(B);
+  d4_sync();
+  d4m_mp<0>(B, M + 0 * 64);
+  d4_sync();
+  d4m_mp<1>(B, M + 1 * 64);  // B = T x0 m0 x1 m1
+  load(A);
+  d4_sync();
+  d4m_mp<3>(A, M + 3 * 64);  // A = T x3 m3
+  d4_sync();
+  d4m_gram_w<2>(A, B, g[2]);
+  d4_sync();
+  load(A);
+  d4_sync();
+  d4m_mp<2>(A, M + 2 * 64);  // A = T x2 m2
+  d4_sync();
+  d4m_gram_w<3>(A, B, g[3]);
+}
+
 __global__ __launch_bounds__(256, 2) void k_bp_d4c8m(ClassView cv, const cplx<float>* __restrict__ T,
                                                   cplx<float>* msgA, cplx<float>* msgB, BpStatus* st) {
   constexpr int H = D4Lds<float>::H, NW = 4;
@@ -680,19 +748,23 @@ inline void launch_bp_d4c8m64(ClassView cv, const cplx<double>* T, cplx<double>*
   hipLaunchKernelGGL((k_bp_d4c8m64<NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, mA, mB, st);
 }
 
-// Extended messages of FP64 jobs at degree 4, chi = 8, on the FP64 MFMA products (k_ext_d4c8m64<NW>):
-// k_ext_d4c8's assembly from the per-half Grams, which here are the waves' 16 x 16 D tiles of
-// d4m64_gram, first reduced over the waves into complex 8 x 8 Grams [k][p] at the start of A.
-template <int NW>
-__global__ __launch_bounds__(64 * NW) void k_ext_d4c8m64(ClassView cv, const cplx<double>* __restrict__ T,
-                                                         const cplx<double>* __restrict__ msgs,
-                                                         cplx<double>* ext_out) {
-  static_assert(NW == 4 || NW == 8, "four or eight waves per node");
-  constexpr int H = D4Lds<double>::H, NT = 64 * NW;
+// Extended messages at degree 4, chi = 8, on MFMA (k_ext_d4c8m<R, NW>): FP64 jobs with d4m64_half,
+// FP32 jobs (four waves) with d4m_ext_half; k_ext_d4c8's assembly from the per-half Grams, which
+// here are the waves' 16 x 16 FP64 D tiles, first reduced over the waves into complex 8 x 8 Grams
+// [k][p] at the start of A.  FP32 jobs run two blocks per CU (66 KB of LDS each; held to 256
+// registers, a spill of about 280 bytes per lane: 100x100 grid step 14.5 -> 13.7 ms against one
+// block per CU without the spill).
+template <typename R, int NW>
+__global__ __launch_bounds__(64 * NW, sizeof(R) == sizeof(float) ? 2 : 1) void k_ext_d4c8m(ClassView cv, const cplx<R>* __restrict__ T,
+                                                       const cplx<R>* __restrict__ msgs, cplx<double>* ext_out) {
+  static_assert(NW == 4 || (NW == 8 && sizeof(R) == sizeof(double)), "four waves (or eight for FP64 jobs)");
+  constexpr int H = D4Lds<R>::H, NT = 64 * NW;
+  // the D tiles [k][p][wave][256] of the reduction fill the A and B buffers at most
+  static_assert(4 * 2 * NW * 256 * (int)sizeof(double) <= 2 * H * (int)sizeof(cplx<R>), "reduction fits A and B");
   extern __shared__ __align__(16) unsigned char d4_lds[];
-  cplx<double>* A = reinterpret_cast<cplx<double>*>(d4_lds);
-  cplx<double>* B = A + H;
-  cplx<double>* M = B + H;
+  cplx<R>* A = reinterpret_cast<cplx<R>*>(d4_lds);
+  cplx<R>* B = A + H;
+  cplx<R>* M = B + H;
   const int n = cv.n;
   const int t = threadIdx.x, o = t & 63;
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
@@ -705,9 +777,14 @@ __global__ __launch_bounds__(64 * NW) void k_ext_d4c8m64(ClassView cv, const cpl
       for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int v = 0; v < 4; ++v) g[p][k][v] = 0.0;
-    d4m64_half<NW>(A, B, M, T + (size_t)i * 2 * H, g[0]);
-    d4_sync();
-    d4m64_half<NW>(A, B, M, T + (size_t)i * 2 * H + H, g[1]);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {  // unrolled: g[p] stays in registers
+      if (p) d4_sync();  // the last Gram of the first half has read B
+      if constexpr (sizeof(R) == sizeof(double))
+        d4m64_half<NW>(A, B, M, T + (size_t)i * 2 * H + (size_t)p * H, g[p]);
+      else
+        d4m_ext_half(A, B, M, T + (size_t)i * 2 * H + (size_t)p * H, g[p]);
+    }
     d4_sync();
     // the D tiles [k][p][wave][256] (16 NW KB, in the dead A and B buffers, contiguous)
     double* red = reinterpret_cast<double*>(A);
@@ -735,7 +812,7 @@ __global__ __launch_bounds__(64 * NW) void k_ext_d4c8m64(ClassView cv, const cpl
       ge[u] = mk(dsum(c, e) + dsum(8 + c, 8 + e), dsum(c, 8 + e) - dsum(8 + c, e));
     }
     d4_sync();
-    cplx<double>* gm = A;  // [k][p][64]
+    cplx<double>* gm = reinterpret_cast<cplx<double>*>(A);  // [k][p][64]
 #pragma unroll
     for (int u = 0; u < PER; ++u) gm[t + NT * u] = ge[u];
     d4_sync();
@@ -756,12 +833,11 @@ __global__ __launch_bounds__(64 * NW) void k_ext_d4c8m64(ClassView cv, const cpl
   }
 }
 
-template <int NW>
-inline void launch_ext_d4c8m64(ClassView cv, const cplx<double>* T, const cplx<double>* msgs, cplx<double>* ext,
-                               hipStream_t s) {
-  constexpr int LDS = D4Lds<double>::BYTES;
-  const int grid = node_grid_lds<k_ext_d4c8m64<NW>, 64 * NW, LDS>(cv.n, 1);
-  hipLaunchKernelGGL((k_ext_d4c8m64<NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, msgs, ext);
+template <typename R, int NW>
+inline void launch_ext_d4c8m(ClassView cv, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext, hipStream_t s) {
+  constexpr int LDS = D4Lds<R>::BYTES;
+  const int grid = node_grid_lds<k_ext_d4c8m<R, NW>, 64 * NW, LDS>(cv.n, 1);
+  hipLaunchKernelGGL((k_ext_d4c8m<R, NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, msgs, ext);
 }
 
 }  // namespace bqa

@@ -66,7 +66,7 @@ inline void launch_mfma16_ext(ClassView cv, const cplx<float>* T, const cplx<flo
 }
 
 // Extended messages in FP64 from R storage: MFMA for chi = 8 and chi = 16 (degree 2, 3), the
-// degree-4 chi = 8 kernels (FP64 MFMA for FP64 jobs), k_small for the other served shapes; false
+// degree-4 chi = 8 MFMA kernel, k_small for the other served shapes; false
 // leaves the generic kernel.  The ZZ-split gates of the step come from cv.zz.
 // FP64, chi = 16, degree 2 or 3: k_mfma16_ext64, one physical half at a time.
 template <int D, int W>
@@ -96,10 +96,7 @@ bool fast_ext_launch(ClassView cv, int chi, const cplx<R>* T, const cplx<R>* msg
     return true;
   }
   if (chi == 8 && cv.d == 4) {
-    if constexpr (sizeof(R) == sizeof(double))
-      launch_ext_d4c8m64<4>(cv, T, msgs, ext, s);
-    else
-      launch_ext_d4c8<R, 512>(cv, T, msgs, ext, s);
+    launch_ext_d4c8m<R, 4>(cv, T, msgs, ext, s);
     return true;
   }
   if (chi == 8 && (cv.d == 2 || cv.d == 3)) {
