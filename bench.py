@@ -330,6 +330,8 @@ def main():
     ap.add_argument("--no-hhex", action="store_true", help="skip the heavy-hex chi=16 line")
     ap.add_argument("--hhex-precisions", default="single,double",
                     help="precisions of the heavy-hex line (comma-separated; the error needs both)")
+    ap.add_argument("--grid-precisions", default=None,
+                    help="precisions of the grid line (comma-separated; default: the timed one, then the other)")
     ap.add_argument("--serve-jobs", type=int, default=48, help="jobs in the serving sample (0 skips it)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="also run the plain copy kernel on BYTES bytes (HBM counter calibration)")
@@ -402,7 +404,9 @@ def main():
         extra["chi4"] = c4
     if not args.no_grid:
         g = {}
-        for prec in (args.precision, other):
+        grid_precs = ([p for p in args.grid_precisions.split(",") if p] if args.grid_precisions
+                      else [args.precision, other])
+        for prec in grid_precs:
             ctx = config_to_context(grid_config(GRID_SIDE, args.chi, BASE_STEPS, prec))
             log(f"[bench] grid {GRID_SIDE}x{GRID_SIDE} chi={args.chi} {TAG[prec]}: {GRID_STEPS} timed steps")
             r = run_anneal(ctx, prec, args.warmup, GRID_STEPS, finish=False)
@@ -424,8 +428,9 @@ def main():
                 f"{HHEX_TIMED} steps timed at chi={HHEX_CHI}")
             r = run_anneal(ctx, prec, args.warmup, HHEX_TIMED, finish=not args.no_accuracy)
             hs = summarize(r, nq, HHEX_TIMED)
-            kern = ("k_mfma16_bp<3,8> / <2,1> (FP32 MFMA v_mfma_f32_16x16x4f32)" if prec == "single"
-                    else "k_node_contract<double> (generic FP64)")
+            kern = ("k_mfma16_bp3h<8> or k_mfma16_bp<3,8> (autotuned) / k_mfma16_bp<2,1> (FP32 MFMA "
+                    "v_mfma_f32_16x16x4f32)" if prec == "single"
+                    else "k_mfma16_bp64<3,8> / <2,1> (FP64 MFMA v_mfma_f64_16x16x4f64)")
             hs["roofline"] = roofline_of(hs, ctx, HHEX_CHI, prec, f"BP sweep at chi={HHEX_CHI}, d=2,3: {kern}")
             hx[TAG[prec]] = hs
             runs[prec] = r
