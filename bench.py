@@ -66,6 +66,7 @@ HHEX_CELLS = 64              # heavy-hex lattice of 64 x 64 cells: 10,144 qubits
 HHEX_CHI = 16
 HHEX_STEPS = 300
 HHEX_TIMED = 10
+HHEX32_TIMED = 3             # chi = 32 runs the generic kernels (seconds per step)
 TAG = {"single": "fp32", "double": "fp64"}
 
 
@@ -328,6 +329,7 @@ def main():
     ap.add_argument("--no-chi4", action="store_true")
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-hhex", action="store_true", help="skip the heavy-hex chi=16 line")
+    ap.add_argument("--no-hhex32", action="store_true", help="skip the heavy-hex chi=32 line")
     ap.add_argument("--hhex-precisions", default="single,double",
                     help="precisions of the heavy-hex line (comma-separated; the error needs both)")
     ap.add_argument("--grid-precisions", default=None,
@@ -445,6 +447,20 @@ def main():
                           f"{HHEX_TIMED} steps timed once the bond dimension is {HHEX_CHI}; fp32 = complex64 "
                           "tensors/messages on FP32 MFMA with FP64 canonicalisers, fp64 = complex128 throughout")
         extra["heavy_hex_chi16"] = hx
+        if not args.no_hhex32:
+            # the same lattice at chi = 32, FP32 storage: no chi = 32 kernels, so the generic node
+            # kernels and the generic canonicaliser run it; a short timed window
+            ctx = config_to_context(hhex_config(32, HHEX_STEPS, "single"))
+            log(f"[bench] heavy-hex {ctx.nodes_number} qubits chi=32 fp32 (generic kernels): {HHEX32_TIMED} steps "
+                f"timed at chi=32")
+            r = run_anneal(ctx, "single", args.warmup, HHEX32_TIMED, finish=False)
+            h32 = summarize(r, ctx.nodes_number, HHEX32_TIMED)
+            h32.pop("full_anneal")
+            h32["workload"] = (f"the heavy-hex lattice above at chi=32, fp32 storage, {HHEX32_TIMED} steps timed once "
+                               "the bond dimension is 32; generic node kernels and canonicaliser")
+            extra["heavy_hex_chi32"] = h32
+            log(f"[bench] heavy-hex chi=32 fp32: {h32['qubit_steps_per_s']:.4g} qubit*steps/s, "
+                f"{h32['ms_per_step']:.1f} ms/step")
     serving = None
     if args.serve_jobs > 0:
         serving = serving_sample(args.serve_jobs, degrees=(3, 4, 5), chi_cap={5: 4})
