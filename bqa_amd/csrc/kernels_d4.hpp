@@ -66,18 +66,16 @@ __device__ __forceinline__ void d4_mp(cplx<C>* X, const cplx<C>* __restrict__ m)
 }
 
 // Gram along bond K: g += sum over this thread's fibre group r of conj(A[r, c]) B[r, e], entry (c, e) =
-// (threadIdx % 64) / 8, % 8; the NT / 64 groups of 64 threads take every (NT / 64)-th fibre.  The
-// products and sums are formed in GC (double for the extended messages of FP32 jobs: exact products
-// of the FP32 operands, FP64 sums).
-template <int K, int NT, typename C, typename GC = C>
+// (threadIdx % 64) / 8, % 8; the NT / 64 groups of 64 threads take every (NT / 64)-th fibre.
+template <int K, int NT, typename C>
 __device__ __forceinline__ void d4_gram(const cplx<C>* A, const cplx<C>* B, cplx<double>& g) {
   constexpr int st = K == 0 ? 512 : K == 1 ? 64 : K == 2 ? 8 : 1;
   const int o = threadIdx.x & 63, grp = threadIdx.x >> 6, c = o >> 3, e = o & 7;
-  cplx<GC> acc = mk<GC>(0, 0);
+  cplx<C> acc = mk<C>(0, 0);
 #pragma unroll 8
   for (int r = grp; r < 512; r += NT / 64) {
     const int base = (r / st) * 8 * st + r % st;
-    cfma_cj(acc, cast<GC>(A[d4_sw(base + c * st)]), cast<GC>(B[d4_sw(base + e * st)]));
+    cfma_cj(acc, A[d4_sw(base + c * st)], B[d4_sw(base + e * st)]);
   }
   g = g + cast<double>(acc);
 }
@@ -87,7 +85,7 @@ __device__ __forceinline__ void d4_sync() { __syncthreads(); }
 // One physical half T_p of a node: the schedule above, adding the half's four Grams into g[k].
 // M holds the four incoming messages; on return A and B are still being read by other threads'
 // last Gram (the caller synchronises before reusing them).
-template <int NT, typename R, typename C, typename GC = C>
+template <int NT, typename R, typename C>
 __device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M, const cplx<R>* __restrict__ Tp,
                                         cplx<double> (&g)[4]) {
   constexpr int H = D4Lds<C>::H;
@@ -105,13 +103,13 @@ __device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M
   d4_sync();
   d4_mp<1, NT, C>(A, M + 1 * 64);   // A = T x1 m1
   d4_sync();
-  d4_gram<0, NT, C, GC>(A, B, g[0]);
+  d4_gram<0, NT, C>(A, B, g[0]);
   d4_sync();
   load(A);
   d4_sync();
   d4_mp<0, NT, C>(A, M + 0 * 64);   // A = T x0 m0
   d4_sync();
-  d4_gram<1, NT, C, GC>(A, B, g[1]);
+  d4_gram<1, NT, C>(A, B, g[1]);
   d4_sync();
   load(B);
   d4_sync();
@@ -122,13 +120,13 @@ __device__ __forceinline__ void d4_half(cplx<C>* A, cplx<C>* B, const cplx<C>* M
   d4_sync();
   d4_mp<3, NT, C>(A, M + 3 * 64);   // A = T x3 m3
   d4_sync();
-  d4_gram<2, NT, C, GC>(A, B, g[2]);
+  d4_gram<2, NT, C>(A, B, g[2]);
   d4_sync();
   load(A);
   d4_sync();
   d4_mp<2, NT, C>(A, M + 2 * 64);   // A = T x2 m2
   d4_sync();
-  d4_gram<3, NT, C, GC>(A, B, g[3]);
+  d4_gram<3, NT, C>(A, B, g[3]);
 }
 
 // NT threads per node (256 or 512: one or two waves per SIMD, the node filling the CU's LDS in FP64).
@@ -208,77 +206,6 @@ inline void launch_bp_d4c8(ClassView cv, const cplx<R>* T, cplx<R>* mA, cplx<R>*
   hipLaunchKernelGGL((k_bp_d4c8<R, NT>), dim3(grid), dim3(NT), LDS, s, cv, T, mA, mB, st);
 }
 
-
-// Extended messages of degree-4 nodes at chi = 8 (state.py:127-139, backends.py:393-404,519-526): the
-// mode products in the storage precision R and the Grams in FP64 (for FP32 jobs, exact products of the
-// FP32 operands with FP64 sums, as the chi = 8 MFMA kernels do), the same products per physical half,
-// the Grams kept per half (p), and each
-// 16 x 16 extended message assembled from the two halves and the bond's ZZ-split gates g0, g1:
-// ext[(s,c),(s',e)] = conj(g_s) g_s' (G_0 +- G_1)[c][e] / (trace * (|g0|^2 + |g1|^2)), with + for
-// s + s' even.  One node per block, NT threads.
-template <typename R, int NT>
-__global__ __launch_bounds__(NT) void k_ext_d4c8(ClassView cv, const cplx<R>* __restrict__ T,
-                                                 const cplx<R>* __restrict__ msgs, cplx<double>* ext_out) {
-  static_assert(NT == 256 || NT == 512, "four or eight waves per node");
-  using C = R;
-  constexpr int H = D4Lds<C>::H, NG = NT / 64;
-  extern __shared__ __align__(16) unsigned char d4_lds[];
-  cplx<C>* A = reinterpret_cast<cplx<C>*>(d4_lds);
-  cplx<C>* B = A + H;
-  cplx<C>* M = B + H;
-  const int n = cv.n;
-  const int t = threadIdx.x, o = t & 63;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) {
-    d4_sync();  // the previous node's last readers of A, B and M are done
-    if (t < 256) {
-      const int j = t >> 6;
-      M[j * 64 + o] = cast<C>(msgs[(size_t)cv.in_pos[j * n + i] * 64 + o]);
-    }
-    cplx<double> g[2][4];
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) g[p][k] = mk(0.0, 0.0);
-    d4_half<NT, R, C, double>(A, B, M, T + (size_t)i * 2 * H, g[0]);
-    d4_sync();
-    d4_half<NT, R, C, double>(A, B, M, T + (size_t)i * 2 * H + H, g[1]);
-    d4_sync();
-    // [k][p][grp][64]: at most 64 KB, the dead A and B buffers (contiguous, 64 KB in FP32)
-    cplx<double>* red = reinterpret_cast<cplx<double>*>(A);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int p = 0; p < 2; ++p) red[((k * 2 + p) * NG + (t >> 6)) * 64 + o] = g[p][k];
-    d4_sync();
-    for (int x = t; x < 4 * 256; x += NT) {
-      const int k = x >> 8, e16 = x & 255, row = e16 >> 4, col = e16 & 15;
-      const int s = row >> 3, c = row & 7, s2 = col >> 3, e = col & 7;
-      auto sum = [&](int p, int entry) {
-        cplx<double> v = mk(0.0, 0.0);
-#pragma unroll
-        for (int q = 0; q < NG; ++q) v = v + red[((k * 2 + p) * NG + q) * 64 + entry];
-        return v;
-      };
-      cplx<double> tr = mk(0.0, 0.0);
-#pragma unroll
-      for (int d = 0; d < 8; ++d) tr = tr + sum(0, d * 9) + sum(1, d * 9);
-      const cplx<double>* zg = cv.zz + 2 * (size_t)cv.lmbd_pos[k * n + i];
-      const cplx<double> g0 = zg[0], g1 = zg[1];
-      const cplx<double> inv = cinv(tr * mk(abs2(g0) + abs2(g1), 0.0));
-      const cplx<double> a0 = sum(0, c * 8 + e), a1 = sum(1, c * 8 + e);
-      const cplx<double> gs = ((s + s2) & 1) ? a0 - a1 : a0 + a1;
-      const cplx<double> ga = s ? g1 : g0, gb = s2 ? g1 : g0;
-      ext_out[(size_t)cv.out_pos[k * n + i] * 256 + e16] = conj(ga) * gb * gs * inv;
-    }
-  }
-}
-
-template <typename R, int NT>
-inline void launch_ext_d4c8(ClassView cv, const cplx<R>* T, const cplx<R>* msgs, cplx<double>* ext, hipStream_t s) {
-  constexpr int LDS = D4Lds<R>::BYTES;
-  const int grid = node_grid_lds<k_ext_d4c8<R, NT>, NT, LDS>(cv.n, 1);
-  hipLaunchKernelGGL((k_ext_d4c8<R, NT>), dim3(grid), dim3(NT), LDS, s, cv, T, msgs, ext);
-}
 
 // ---------------------------------------------------------------------------
 // The same sweep for FP32 jobs on v_mfma_f32_16x16x4f32 (k_bp_d4c8m; four waves per node, same
@@ -748,10 +675,13 @@ inline void launch_bp_d4c8m64(ClassView cv, const cplx<double>* T, cplx<double>*
   hipLaunchKernelGGL((k_bp_d4c8m64<NW>), dim3(grid), dim3(64 * NW), LDS, s, cv, T, mA, mB, st);
 }
 
-// Extended messages at degree 4, chi = 8, on MFMA (k_ext_d4c8m<R, NW>): FP64 jobs with d4m64_half,
-// FP32 jobs (four waves) with d4m_ext_half; k_ext_d4c8's assembly from the per-half Grams, which
-// here are the waves' 16 x 16 FP64 D tiles, first reduced over the waves into complex 8 x 8 Grams
-// [k][p] at the start of A.  FP32 jobs run two blocks per CU (66 KB of LDS each; held to 256
+// Extended messages of degree-4 nodes at chi = 8 (state.py:127-139, backends.py:393-404,519-526) on
+// MFMA (k_ext_d4c8m<R, NW>): the BP kernels' products per physical half p (FP64 jobs: d4m64_half;
+// FP32 jobs, four waves: d4m_ext_half, FP32 mode products and FP64 Grams), with the Grams kept per
+// half.  The waves' 16 x 16 FP64 D tiles are reduced over the waves into complex 8 x 8 Grams G_p[k]
+// at the start of A, and each 16 x 16 extended message is assembled with the bond's ZZ-split gates
+// g0, g1: ext[(s,c),(s',e)] = conj(g_s) g_s' (G_0 +- G_1)[c][e] / (trace * (|g0|^2 + |g1|^2)), with
+// + for s + s' even.  FP32 jobs run two blocks per CU (66 KB of LDS each; held to 256
 // registers, a spill of about 280 bytes per lane: 100x100 grid step 14.5 -> 13.7 ms against one
 // block per CU without the spill).
 template <typename R, int NW>
