@@ -1,8 +1,10 @@
-// BP sweep for degree-4 nodes at chi = 8 (the 100x100 grid's interior, the d = 4
-// serving jobs).  A node tensor is 2 x 8^4 complex elements (128 KB in FP64), so
-// the kernel works on one physical half T_p (4096 elements) at a time, in two
-// LDS buffers A and B, with mode products done in place (each thread owns whole
-// fibres along the product's bond, reads them, then writes them back).
+// BP sweep and extended messages for degree-4 nodes at chi = 8 (the 100x100
+// grid's interior, the d = 4 serving jobs): k_bp_d4c8 on VALU, k_bp_d4c8m (FP32)
+// and k_bp_d4c8m64 (FP64) on MFMA, k_ext_d4c8m on MFMA.  A node tensor is
+// 2 x 8^4 complex elements (128 KB in FP64), so the kernels work on one physical
+// half T_p (4096 elements) at a time, in two LDS buffers A and B, with mode
+// products done in place (a thread or a wave owns whole fibres along the
+// product's bond, reads them, then writes them back).
 //
 // The reference computes out_k = Gram over every bond but k of conj(T) against
 // T contracted with the three other incoming messages (backends.py:381-404).
