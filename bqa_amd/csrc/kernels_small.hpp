@@ -419,11 +419,12 @@ __global__ __launch_bounds__(64) void k_small_multi(const BpEntry* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // BP sweep with NPW nodes per wave (chi <= 4, degree 2 and 3): each node gets
-// GL = 64 / NPW lanes, its own LDS tensor and work buffers, and the same
-// schedule as k_small; the wave's phases (and their barriers) serve NPW nodes
-// at once, and at GL >= Q a Gram entry is summed by one lane over all fibres
-// (no cross-lane reduction).  Same arithmetic per node as k_small, in the same
-// order within each lane except the Gram's fibre split.
+// GL = 64 / NPW lanes and its own LDS tensor and work buffers; the wave's phases
+// (and their barriers) serve NPW nodes at once, and at GL >= Q a Gram entry is
+// summed by one lane over all fibres (no cross-lane reduction).  Degree 2 has
+// k_small's schedule; degree 3 moves one message of each output to the bra side
+// (three mode products instead of five), so its messages equal k_small's to
+// rounding.
 template <int D, int CHI, int NPW, typename C>
 struct NpwSmem {
   cplx<C> T[NPW][Sm<D, CHI>::S];
@@ -479,12 +480,36 @@ __device__ __forceinline__ cplx<C> sgram_g(const cplx<C>* T, const cplx<C>* Y, i
   return g;
 }
 
+// Y = X x_{D-1} m in place along the last bond: each lane owns whole fibres of this group's node
+// (reads its fibres, then writes them), so no other lane's reads are affected.
+template <int D, int CHI, int GL, typename C>
+__device__ __forceinline__ void smp_g_last_inplace(cplx<C>* X, const cplx<C>* m, int gl) {
+  constexpr int RT = Sm<D, CHI>::RT;
+#pragma unroll
+  for (int q = 0; q < (RT + GL - 1) / GL; ++q) {
+    const int f = gl + GL * q;
+    if (RT % GL == 0 || f < RT) {
+      cplx<C> x[CHI];
+#pragma unroll
+      for (int b = 0; b < CHI; ++b) x[b] = X[f * CHI + b];
+#pragma unroll
+      for (int a = 0; a < CHI; ++a) {
+        cplx<C> acc = mk<C>(0, 0);
+#pragma unroll
+        for (int b = 0; b < CHI; ++b) cfma(acc, m[a * CHI + b], x[b]);
+        X[f * CHI + a] = acc;
+      }
+    }
+  }
+}
+
+// Output message K of this group's node from the Gram sum over every bond but K of conj(A) B.
 template <int D, int CHI, int K, int NPW, typename R, typename C>
-__device__ __forceinline__ void sout_g(NpwSmem<D, CHI, NPW, C>& sm, const cplx<C>* Y, const ClassView& cv, int i,
-                                       bool valid, int gi, int gl, const cplx<R> (&old)[D], cplx<R>* mout,
-                                       double& dnum, double& dden) {
+__device__ __forceinline__ void sout_g(NpwSmem<D, CHI, NPW, C>& sm, const cplx<C>* A, const cplx<C>* Y,
+                                       const ClassView& cv, int i, bool valid, int gi, int gl,
+                                       const cplx<R> (&old)[D], cplx<R>* mout, double& dnum, double& dden) {
   constexpr int GL = 64 / NPW, Q = CHI * CHI;
-  const cplx<C> g = sgram_g<D, CHI, K, GL, C>(sm.T[gi], Y, gl);
+  const cplx<C> g = sgram_g<D, CHI, K, GL, C>(A, Y, gl);
   if (gl < Q) sm.red[gi][gl] = g;
   small_sync();
   if (valid && gl < Q) {
@@ -542,24 +567,23 @@ __device__ __forceinline__ void npw_run(NpwSmem<D, CHI, NPW, R>& sm, const Class
     if constexpr (D == 2) {
       smp_g<D, CHI, 1, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][1], gl);  // Y0 = T x1 m1
       small_sync();
-      sout_g<D, CHI, 0, NPW, R, C>(sm, sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      sout_g<D, CHI, 0, NPW, R, C>(sm, sm.T[gi], sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
       smp_g<D, CHI, 0, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][0], gl);  // Y1 = T x0 m0
       small_sync();
-      sout_g<D, CHI, 1, NPW, R, C>(sm, sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      sout_g<D, CHI, 1, NPW, R, C>(sm, sm.T[gi], sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
     } else {
-      smp_g<D, CHI, 2, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][2], gl);  // W = T x2 m2
+      // The incoming messages are Hermitian, so one moves to the bra side as a product of the
+      // conjugated factor: with P_j = T x_j m_j, out_0 = sum conj(P_2) P_1, out_1 = sum conj(P_2) P_0,
+      // out_2 = sum conj(P_1) P_0 (each over the other bonds): three mode products instead of the
+      // reference schedule's five (backends.py:381-404), P_2 in place of T.
+      smp_g<D, CHI, 0, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][0], gl);  // P_0 = T x0 m0
+      smp_g<D, CHI, 1, GL, C>(sm.T[gi], sm.Y[gi], sm.m[gi][1], gl);  // P_1 = T x1 m1
       small_sync();
-      smp_g<D, CHI, 1, GL, C>(sm.X[gi], sm.Y[gi], sm.m[gi][1], gl);  // Y0 = W x1 m1
+      sout_g<D, CHI, 2, NPW, R, C>(sm, sm.Y[gi], sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      smp_g_last_inplace<D, CHI, GL, C>(sm.T[gi], sm.m[gi][2], gl);  // T <- P_2 = T x2 m2
       small_sync();
-      sout_g<D, CHI, 0, NPW, R, C>(sm, sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
-      smp_g<D, CHI, 0, GL, C>(sm.X[gi], sm.Y[gi], sm.m[gi][0], gl);  // Y1 = W x0 m0
-      small_sync();
-      sout_g<D, CHI, 1, NPW, R, C>(sm, sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
-      smp_g<D, CHI, 0, GL, C>(sm.T[gi], sm.X[gi], sm.m[gi][0], gl);  // W = T x0 m0
-      small_sync();
-      smp_g<D, CHI, 1, GL, C>(sm.X[gi], sm.Y[gi], sm.m[gi][1], gl);  // Y2 = W x1 m1
-      small_sync();
-      sout_g<D, CHI, 2, NPW, R, C>(sm, sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      sout_g<D, CHI, 0, NPW, R, C>(sm, sm.T[gi], sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
+      sout_g<D, CHI, 1, NPW, R, C>(sm, sm.T[gi], sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
