@@ -75,6 +75,31 @@ __device__ __forceinline__ void smp(const cplx<C>* X, cplx<C>* Y, const cplx<C>*
   }
 }
 
+// X <- X x_{D-1} m in place along the last bond, over GL lanes (gl: this lane's index among them):
+// each lane owns whole fibres (reads its fibres, then writes them), so no other lane's reads are
+// affected.
+template <int D, int CHI, int GL, typename C>
+__device__ __forceinline__ void smp_last_inplace(cplx<C>* X, const cplx<C>* m, int gl) {
+  constexpr int RT = Sm<D, CHI>::RT;
+#pragma unroll
+  for (int q = 0; q < (RT + GL - 1) / GL; ++q) {
+    const int f = gl + GL * q;
+    if (RT % GL == 0 || f < RT) {
+      cplx<C> x[CHI];
+#pragma unroll
+      for (int b = 0; b < CHI; ++b) x[b] = X[f * CHI + b];
+#pragma unroll
+      for (int a = 0; a < CHI; ++a) {
+        cplx<C> acc = mk<C>(0, 0);
+#pragma unroll
+        for (int b = 0; b < CHI; ++b) cfma(acc, m[a * CHI + b], x[b]);
+        X[f * CHI + a] = acc;
+      }
+    }
+  }
+}
+
+
 // Gram along bond K: G[c][e] = sum over fibres r of conj(T[r, c]) Y[r, e], with
 // PSPLIT separate sums over the p = 0 and p = 1 halves of the fibres (p is the
 // slowest index).  Lane l takes entry o = l % Q and the fibres r = g mod NG,
@@ -193,12 +218,12 @@ using SmallGC = typename std::conditional<MODE == kModeExt, double, C>::type;
 
 // Outgoing message (BP) or extended message (Ext) along bond K from Y_K.
 template <int D, int CHI, int K, int MODE, typename R, typename C>
-__device__ __forceinline__ void sout(SmallSmem<D, CHI, C, SmallGC<MODE, C>>& sm, const cplx<C>* Y,
-                                     const ClassView& cv, int i, const cplx<R> (&old)[D], cplx<R>* mout,
-                                     cplx<double>* ext_out, double& dnum, double& dden) {
+__device__ __forceinline__ void sout(SmallSmem<D, CHI, C, SmallGC<MODE, C>>& sm, const cplx<C>* A,
+                                     const cplx<C>* Y, const ClassView& cv, int i, const cplx<R> (&old)[D],
+                                     cplx<R>* mout, cplx<double>* ext_out, double& dnum, double& dden) {
   using GC = SmallGC<MODE, C>;
   cplx<GC> g[2];
-  sgram<D, CHI, K, MODE == kModeExt, C, GC>(sm.T, Y, sm.red, g);
+  sgram<D, CHI, K, MODE == kModeExt, C, GC>(A, Y, sm.red, g);
   const long pos = cv.out_pos[K * cv.n + i];
   if constexpr (MODE == kModeBP) {
     sfinish_bp<CHI, R, C>(g, sm.red, pos, old[K], mout, dnum, dden);
@@ -302,24 +327,24 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
     } else if constexpr (D == 2) {
       smp<D, CHI, 1, C>(sm.T, sm.X, sm.m[1]);  // Y0 = T x1 m1
       small_sync();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // Y1 = T x0 m0
       small_sync();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     } else if constexpr (D == 3) {
-      smp<D, CHI, 2, C>(sm.T, sm.X, sm.m[2]);  // W = T x2 m2
+      // The incoming messages are Hermitian, so one moves to the bra side as a product of the
+      // conjugated factor: with P_j = T x_j m_j, out_0 = sum conj(P_2) P_1, out_1 = sum conj(P_2) P_0,
+      // out_2 = sum conj(P_1) P_0 (each over the other bonds, per physical half for the extended
+      // messages): three mode products instead of the reference schedule's five
+      // (backends.py:381-404), P_2 in place of T.
+      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // P_0 = T x0 m0
+      smp<D, CHI, 1, C>(sm.T, sm.Y, sm.m[1]);  // P_1 = T x1 m1
       small_sync();
-      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y0 = W x1 m1
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      smp_last_inplace<D, CHI, 64, C>(sm.T, sm.m[2], threadIdx.x & 63);  // T <- P_2 = T x2 m2
       small_sync();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
-      smp<D, CHI, 0, C>(sm.X, sm.Y, sm.m[0]);  // Y1 = W x0 m0
-      small_sync();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
-      smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);  // W = T x0 m0
-      small_sync();
-      smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y2 = W x1 m1
-      small_sync();
-      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.T, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     } else if constexpr (D == 4) {
       // degree 4: W = T x3 m3 x2 m2 shared by outputs 0 and 1, V = T x0 m0 x1 m1
       // shared by outputs 2 and 3 (8 mode products instead of 12).
@@ -329,20 +354,20 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
       small_sync();
       smp<D, CHI, 1, C>(sm.Y, sm.X, sm.m[1]);  // Y0 = W x1 m1
       small_sync();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.Y, sm.X, sm.m[0]);  // Y1 = W x0 m0
       small_sync();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
       small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // V in Y
       small_sync();
       smp<D, CHI, 3, C>(sm.Y, sm.X, sm.m[3]);  // Y2 = V x3 m3
       small_sync();
-      sout<D, CHI, 2, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 2, C>(sm.Y, sm.X, sm.m[2]);  // Y3 = V x2 m2
       small_sync();
-      sout<D, CHI, 3, MODE, R, C>(sm, sm.X, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 3, MODE, R, C>(sm, sm.T, sm.X, cv, i, old, mout, ext_out, dnum, dden);
     } else {
       // degree 5: A = T x2 m2 x3 m3 x4 m4 shared by outputs 0 and 1, B = T x0 m0 x1 m1 x2 m2
       // shared by outputs 3 and 4, output 2 on its own (14 mode products instead of 20).
@@ -354,10 +379,10 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
       small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);  // Y0 = A x1 m1
       small_sync();
-      sout<D, CHI, 0, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 0, MODE, R, C>(sm, sm.T, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.X, sm.Y, sm.m[0]);  // Y1 = A x0 m0
       small_sync();
-      sout<D, CHI, 1, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 1, MODE, R, C>(sm, sm.T, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
       small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);
@@ -366,7 +391,7 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
       small_sync();
       smp<D, CHI, 4, C>(sm.X, sm.Y, sm.m[4]);  // Y2 = T x0 m0 x1 m1 x3 m3 x4 m4
       small_sync();
-      sout<D, CHI, 2, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 2, MODE, R, C>(sm, sm.T, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 0, C>(sm.T, sm.X, sm.m[0]);
       small_sync();
       smp<D, CHI, 1, C>(sm.X, sm.Y, sm.m[1]);
@@ -375,10 +400,10 @@ __device__ __forceinline__ void small_run(const ClassView& cv, const cplx<R>* __
       small_sync();
       smp<D, CHI, 4, C>(sm.X, sm.Y, sm.m[4]);  // Y3 = B x4 m4
       small_sync();
-      sout<D, CHI, 3, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 3, MODE, R, C>(sm, sm.T, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
       smp<D, CHI, 3, C>(sm.X, sm.Y, sm.m[3]);  // Y4 = B x3 m3
       small_sync();
-      sout<D, CHI, 4, MODE, R, C>(sm, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
+      sout<D, CHI, 4, MODE, R, C>(sm, sm.T, sm.Y, cv, i, old, mout, ext_out, dnum, dden);
     }
   }
   if (MODE == kModeBP && start < n) {  // a block without nodes here adds nothing to the maxima
@@ -480,29 +505,6 @@ __device__ __forceinline__ cplx<C> sgram_g(const cplx<C>* T, const cplx<C>* Y, i
   return g;
 }
 
-// Y = X x_{D-1} m in place along the last bond: each lane owns whole fibres of this group's node
-// (reads its fibres, then writes them), so no other lane's reads are affected.
-template <int D, int CHI, int GL, typename C>
-__device__ __forceinline__ void smp_g_last_inplace(cplx<C>* X, const cplx<C>* m, int gl) {
-  constexpr int RT = Sm<D, CHI>::RT;
-#pragma unroll
-  for (int q = 0; q < (RT + GL - 1) / GL; ++q) {
-    const int f = gl + GL * q;
-    if (RT % GL == 0 || f < RT) {
-      cplx<C> x[CHI];
-#pragma unroll
-      for (int b = 0; b < CHI; ++b) x[b] = X[f * CHI + b];
-#pragma unroll
-      for (int a = 0; a < CHI; ++a) {
-        cplx<C> acc = mk<C>(0, 0);
-#pragma unroll
-        for (int b = 0; b < CHI; ++b) cfma(acc, m[a * CHI + b], x[b]);
-        X[f * CHI + a] = acc;
-      }
-    }
-  }
-}
-
 // Output message K of this group's node from the Gram sum over every bond but K of conj(A) B.
 template <int D, int CHI, int K, int NPW, typename R, typename C>
 __device__ __forceinline__ void sout_g(NpwSmem<D, CHI, NPW, C>& sm, const cplx<C>* A, const cplx<C>* Y,
@@ -580,7 +582,7 @@ __device__ __forceinline__ void npw_run(NpwSmem<D, CHI, NPW, R>& sm, const Class
       smp_g<D, CHI, 1, GL, C>(sm.T[gi], sm.Y[gi], sm.m[gi][1], gl);  // P_1 = T x1 m1
       small_sync();
       sout_g<D, CHI, 2, NPW, R, C>(sm, sm.Y[gi], sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
-      smp_g_last_inplace<D, CHI, GL, C>(sm.T[gi], sm.m[gi][2], gl);  // T <- P_2 = T x2 m2
+      smp_last_inplace<D, CHI, GL, C>(sm.T[gi], sm.m[gi][2], gl);  // T <- P_2 = T x2 m2
       small_sync();
       sout_g<D, CHI, 0, NPW, R, C>(sm, sm.T[gi], sm.Y[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
       sout_g<D, CHI, 1, NPW, R, C>(sm, sm.T[gi], sm.X[gi], cv, ic, valid, gi, gl, old, mout, dnum, dden);
